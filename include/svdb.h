@@ -1,0 +1,127 @@
+/*
+ * svdb.h — batch entry points of the MI355X engine behind simple-vector-db.
+ *
+ * These have no counterpart in the reference; they are the batched forms of
+ * its per-request calls, for callers that can hand over many queries or
+ * pairs at once:
+ *   svdb_search        batched top-k find-nearest with a metric per query —
+ *                      the reference answers one query, k = 1, Euclidean
+ *                      only (kdtree_nearest, kdtree.c:171-178, called from
+ *                      compare_handler.c:403)
+ *   svdb_pair_metrics  batched FP64 cosine / Euclidean / dot over stored rows
+ *                      (the reference computes one float pair per request,
+ *                      vector_database.c:301-352 via compare_handler.c:152-170)
+ *   svdb_bulk_insert   bulk load (the reference inserts row by row,
+ *                      vector_database.c:81-119, 277-279)
+ *
+ * Ranking (the "FP64 oracle" contract): Euclidean ascending on the squared
+ * distance Σ fma((x_i - q_i),(x_i - q_i),·) accumulated in index order;
+ * dot descending on Σ fma(q_i, x_i, ·); cosine descending on
+ * dot / (sqrt(‖q‖²) * sqrt(‖x‖²)) with the norms accumulated the same way.
+ * Equal keys go to the lowest index; NaN ranks after every number.  Returned
+ * scores: Euclidean distance (sqrt of the key), dot product, cosine.  Every
+ * precision mode returns exactly these ids and FP64 scores; the low-precision
+ * modes only change how candidates are found (a certified BF16/FP8 MFMA
+ * first pass, then an FP64 rerank; uncertified queries are rescanned in
+ * FP64).
+ *
+ * All functions return 0 on success or a negative SVDB_E* code;
+ * svdb_last_error() describes the last failure on the calling thread.
+ */
+#ifndef SVDB_H
+#define SVDB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vector_database.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SVDB_METRIC_L2 = 0, SVDB_METRIC_COS = 1, SVDB_METRIC_DOT = 2 };
+
+enum {
+    SVDB_PREC_FP64 = 0, /* plain FP64 exhaustive scan */
+    SVDB_PREC_BF16 = 1, /* BF16 MFMA first pass + shortlist + FP64 rerank */
+    SVDB_PREC_FP8 = 2,  /* FP8-e4m3 MFMA first pass + shortlist + FP64 rerank */
+    SVDB_PREC_AUTO = 3  /* BF16 when the store has a shadow, else FP64 */
+};
+
+enum {
+    SVDB_OK = 0,
+    SVDB_EINVAL = -1,  /* bad argument */
+    SVDB_ENOMEM = -2,  /* device allocation failed or HBM budget exceeded */
+    SVDB_EDEVICE = -3, /* HIP runtime error / no usable GPU */
+    SVDB_EEMPTY = -4,  /* store has no rows */
+    SVDB_EBUSY = -5    /* server queue full */
+};
+
+/* Largest k accepted by the search entry points. */
+#define SVDB_MAX_K 128
+
+/* Top-k for nq host queries (row i at queries + i*dim).  k[i] in
+ * [1, SVDB_MAX_K] and metric[i] per query; out_stride >= max k[i].
+ * ids_out[i*out_stride + j], scores_out[...]: j-th best; slots past the
+ * number of stored rows get (size_t)-1 and NaN.  dim must equal the store's
+ * vector dimension. */
+int svdb_search(VectorDatabase* db, const double* queries, size_t nq, size_t dim,
+                const size_t* k, const int* metric, int precision, size_t out_stride,
+                size_t* ids_out, double* scores_out);
+
+/* Same, with queries, ids and scores already in device memory (HBM):
+ * d_queries[i*ldq + c], d_ids[i*k + j], d_scores[i*k + j]; one k and metric
+ * for the batch.  Work is enqueued on the engine's stream and the call
+ * returns when it has completed. */
+int svdb_search_device(VectorDatabase* db, const double* d_queries, size_t nq, size_t ldq,
+                       size_t k, int metric, int precision, uint64_t* d_ids, double* d_scores);
+
+/* FP64 metric for npairs pairs of stored rows (logical indices), one 64-lane
+ * wavefront per pair: lane l sums elements i ≡ l (mod 64) with fma, then an
+ * xor butterfly.  Euclidean returns the distance. */
+int svdb_pair_metrics(VectorDatabase* db, const size_t* index_a, const size_t* index_b,
+                      size_t npairs, int metric, double* out);
+
+/* Append n rows (row i at rows + i*dim) in one device pass.  uuids: n
+ * NUL-padded 37-byte records, or NULL for empty uuids.  The host mirror gets
+ * malloc'ed copies.  *first_index receives the first new row's index. */
+int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t dim,
+                     const char* uuids, size_t* first_index);
+
+/* Engine knobs. */
+int svdb_set_shortlist(VectorDatabase* db, size_t width); /* first-pass shortlist L (>= k) */
+int svdb_set_budget_bytes(VectorDatabase* db, size_t bytes); /* device-memory budget */
+int svdb_enable_shadow(VectorDatabase* db, int precision); /* build BF16/FP8 shadow */
+
+typedef struct svdb_stats {
+    size_t rows;            /* logical rows */
+    size_t dim;             /* vector dimension (0 before the first insert) */
+    size_t phys_rows;       /* device slots in use (rows + tombstones) */
+    size_t device_bytes;    /* device memory held by the engine */
+    size_t budget_bytes;    /* device-memory budget */
+    uint64_t searches;      /* search calls */
+    uint64_t queries;       /* queries answered */
+    uint64_t certified;     /* low-precision queries certified exact */
+    uint64_t fallbacks;     /* low-precision queries rescanned in FP64 */
+    double last_max_rel_err;/* max |first-pass score - FP64 score| / |FP64| of the last call */
+} svdb_stats;
+int svdb_get_stats(VectorDatabase* db, svdb_stats* out);
+
+/* Kernel timing, measured with HIP events on the engine's stream. */
+typedef struct svdb_kernel_time {
+    uint64_t launches;
+    double total_ms;
+} svdb_kernel_time;
+enum { SVDB_K_SCAN_FP64 = 0, SVDB_K_SCAN_LOWP = 1, SVDB_K_SHORTLIST = 2, SVDB_K_RERANK = 3,
+       SVDB_K_MERGE = 4, SVDB_K_COUNT = 5 };
+int svdb_set_profiling(VectorDatabase* db, int on); /* also resets the counters */
+int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_COUNT] */);
+
+const char* svdb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVDB_H */

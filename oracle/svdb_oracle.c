@@ -301,7 +301,8 @@ static void topk_emit(const topk_t* t, size_t k, int metric, uint64_t* ids, doub
     for (size_t j = 0; j < k; j++) {
         if (j < t->cnt) {
             ids[j] = t->id[j];
-            sc[j] = metric == ORC_L2 ? sqrt(t->key[j]) : -t->key[j];
+            /* every NaN score is emitted as the canonical quiet NaN */
+            sc[j] = isnan(t->key[j]) ? NAN : metric == ORC_L2 ? sqrt(t->key[j]) : -t->key[j];
         } else {
             ids[j] = UINT64_MAX;
             sc[j] = NAN;

@@ -1,0 +1,608 @@
+// svdb_engine.cpp — device store, search pipeline and pair metrics.
+#include "svdb_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace svdb {
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  g_err = msg;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+#define HIPCHK(expr, what)                                                            \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      set_error(SVDB_EDEVICE, std::string(what) + ": " + hipGetErrorString(e_));      \
+      return SVDB_EDEVICE;                                                            \
+    }                                                                                 \
+  } while (0)
+
+static inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// ------------------------------------------------------------------ device
+
+static std::once_flag g_dev_once;
+static int g_dev = -1;
+
+int device_init() {
+  std::call_once(g_dev_once, [] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return;
+    const char* e = std::getenv("SVDB_DEVICE");
+    int d = e ? std::atoi(e) : 0;
+    g_dev = (d >= 0 && d < n) ? d : 0;
+  });
+  if (g_dev < 0) {
+    set_error(SVDB_EDEVICE, "no usable HIP device (libsvdb needs an MI355X / gfx950 GPU)");
+    return SVDB_EDEVICE;
+  }
+  if (hipSetDevice(g_dev) != hipSuccess) {
+    set_error(SVDB_EDEVICE, "hipSetDevice failed");
+    return SVDB_EDEVICE;
+  }
+  return g_dev;
+}
+
+// ------------------------------------------------------------------ legacy metrics
+
+namespace {
+struct RefCtx {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  double* buf = nullptr;
+  size_t cap = 0;  // doubles
+  uint32_t* slots = nullptr;
+  float* out = nullptr;
+};
+RefCtx g_ref;
+}  // namespace
+
+int pair_ref32_host(const double* a, const double* b, size_t n, int metric, float* out) {
+  if (device_init() < 0) return SVDB_EDEVICE;
+  std::lock_guard<std::mutex> g(g_ref.mu);
+  if (!g_ref.stream) {
+    HIPCHK(hipStreamCreateWithFlags(&g_ref.stream, hipStreamNonBlocking), "stream");
+    HIPCHK(hipMalloc(&g_ref.slots, 2 * sizeof(uint32_t)), "hipMalloc");
+    HIPCHK(hipMalloc(&g_ref.out, sizeof(float)), "hipMalloc");
+    uint32_t s[2] = {0, 1};
+    HIPCHK(hipMemcpy(g_ref.slots, s, sizeof(s), hipMemcpyHostToDevice), "copy");
+  }
+  size_t ld = std::max<size_t>(n, 1);
+  if (2 * ld > g_ref.cap) {
+    if (g_ref.buf) (void)hipFree(g_ref.buf);
+    g_ref.buf = nullptr;
+    HIPCHK(hipMalloc(&g_ref.buf, 2 * ld * sizeof(double)), "hipMalloc");
+    g_ref.cap = 2 * ld;
+  }
+  if (n) {
+    HIPCHK(hipMemcpyAsync(g_ref.buf, a, n * sizeof(double), hipMemcpyHostToDevice, g_ref.stream), "copy");
+    HIPCHK(hipMemcpyAsync(g_ref.buf + ld, b, n * sizeof(double), hipMemcpyHostToDevice, g_ref.stream), "copy");
+  }
+  HIPCHK(launch_pair_ref32(g_ref.buf, ld, g_ref.slots, g_ref.slots + 1, 1, (uint32_t)n, metric,
+                           g_ref.out, g_ref.stream), "k_pair_ref32");
+  HIPCHK(hipMemcpyAsync(out, g_ref.out, sizeof(float), hipMemcpyDeviceToHost, g_ref.stream), "copy");
+  HIPCHK(hipStreamSynchronize(g_ref.stream), "sync");
+  return SVDB_OK;
+}
+
+// ------------------------------------------------------------------ engine
+
+Engine::Engine(size_t initial_slots) {
+  if (device_init() < 0) return;
+  device_ = g_dev;
+  if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) {
+    set_error(SVDB_EDEVICE, "hipStreamCreate failed");
+    return;
+  }
+  init_slots_ = round_up(std::max<size_t>(initial_slots, 1024), 64);
+  ok_ = true;
+}
+
+Engine::~Engine() {
+  if (!ok_) return;
+  (void)hipSetDevice(device_);
+  (void)hipStreamSynchronize(stream_);
+  void* bufs[] = {X_, nx2_, xn_, alive_, rank_, s_q_, s_qn_, s_k_, s_pk_, s_ps_, s_ids_, s_sc_,
+                  s_pair_, s_pout_};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  for (auto& e : ev_pool_) (void)hipEventDestroy(e);
+  for (auto& ps : prof_pending_) {
+    (void)hipEventDestroy(ps.a);
+    (void)hipEventDestroy(ps.b);
+  }
+  (void)hipStreamDestroy(stream_);
+}
+
+int Engine::dev_alloc(void** p, size_t bytes) {
+  if (dev_bytes_ + bytes > budget_) {
+    set_error(SVDB_ENOMEM, "device-memory budget exceeded (" + std::to_string(dev_bytes_ + bytes) +
+                               " > " + std::to_string(budget_) + " bytes)");
+    return SVDB_ENOMEM;
+  }
+  if (hipMalloc(p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+    *p = nullptr;
+    set_error(SVDB_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    return SVDB_ENOMEM;
+  }
+  dev_bytes_ += bytes;
+  return SVDB_OK;
+}
+
+void Engine::dev_free(void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  dev_bytes_ -= std::min(bytes, dev_bytes_);
+}
+
+int Engine::scratch(void** p, size_t* have, size_t need) {
+  if (need <= *have && *p) return SVDB_OK;
+  HIPCHK(hipStreamSynchronize(stream_), "sync");
+  dev_free(*p, *have);
+  *p = nullptr;
+  *have = 0;
+  size_t n = round_up(std::max<size_t>(need, 4096), 4096);
+  int rc = dev_alloc(p, n);
+  if (rc) return rc;
+  *have = n;
+  return SVDB_OK;
+}
+
+int Engine::ensure_dim(size_t d) {
+  if (d == 0) {
+    set_error(SVDB_EINVAL, "vector dimension 0");
+    return SVDB_EINVAL;
+  }
+  if (d_ == 0) {
+    d_ = d;
+    ld_ = round_up(d, 8);
+    return ensure_capacity(init_slots_);
+  }
+  if (d != d_) {
+    set_error(SVDB_EINVAL, "vector dimension " + std::to_string(d) + " differs from the store's " +
+                               std::to_string(d_));
+    return SVDB_EINVAL;
+  }
+  return SVDB_OK;
+}
+
+int Engine::ensure_capacity(size_t need) {
+  if (need <= cap_) return SVDB_OK;
+  if (need >= (size_t)UINT32_MAX - 64) {
+    set_error(SVDB_EINVAL, "store is limited to 2^32-65 slots");
+    return SVDB_EINVAL;
+  }
+  size_t nc = round_up(std::max({need, cap_ * 2, init_slots_}), 64);
+  nc = std::min(nc, round_up((size_t)UINT32_MAX - 128, 64) - 64);
+  const size_t nw = nc / 64, ow = cap_ / 64;
+  double *X = nullptr, *nx2 = nullptr, *xn = nullptr;
+  uint64_t* al = nullptr;
+  uint32_t* rk = nullptr;
+  int rc;
+  if ((rc = dev_alloc((void**)&X, nc * ld_ * sizeof(double)))) return rc;
+  if ((rc = dev_alloc((void**)&nx2, nc * sizeof(double))) ||
+      (rc = dev_alloc((void**)&xn, nc * sizeof(double))) ||
+      (rc = dev_alloc((void**)&al, nw * sizeof(uint64_t))) ||
+      (rc = dev_alloc((void**)&rk, nw * sizeof(uint32_t)))) {
+    dev_free(X, nc * ld_ * sizeof(double));
+    dev_free(nx2, nc * sizeof(double));
+    dev_free(xn, nc * sizeof(double));
+    dev_free(al, nw * sizeof(uint64_t));
+    return rc;
+  }
+  if (nphys_) {
+    HIPCHK(hipMemcpyAsync(X, X_, nphys_ * ld_ * sizeof(double), hipMemcpyDeviceToDevice, stream_), "grow");
+    HIPCHK(hipMemcpyAsync(nx2, nx2_, nphys_ * sizeof(double), hipMemcpyDeviceToDevice, stream_), "grow");
+    HIPCHK(hipMemcpyAsync(xn, xn_, nphys_ * sizeof(double), hipMemcpyDeviceToDevice, stream_), "grow");
+  }
+  HIPCHK(hipStreamSynchronize(stream_), "grow");
+  dev_free(X_, cap_ * ld_ * sizeof(double));
+  dev_free(nx2_, cap_ * sizeof(double));
+  dev_free(xn_, cap_ * sizeof(double));
+  dev_free(alive_, ow * sizeof(uint64_t));
+  dev_free(rank_, ow * sizeof(uint32_t));
+  X_ = X;
+  nx2_ = nx2;
+  xn_ = xn;
+  alive_ = al;
+  rank_ = rk;
+  cap_ = nc;
+  h_alive_.resize(nw, 0);
+  h_rank_.resize(nw, 0);
+  dirty_word_ = 0;  // the new bitmap arrays hold nothing yet
+  return SVDB_OK;
+}
+
+int Engine::sync_alive() {
+  const size_t nw = (nphys_ + 63) / 64;
+  if (dirty_word_ >= nw) {
+    dirty_word_ = SIZE_MAX;
+    return SVDB_OK;
+  }
+  const size_t w0 = dirty_word_, n = nw - w0;
+  HIPCHK(hipMemcpyAsync(alive_ + w0, h_alive_.data() + w0, n * sizeof(uint64_t),
+                        hipMemcpyHostToDevice, stream_), "alive upload");
+  HIPCHK(hipMemcpyAsync(rank_ + w0, h_rank_.data() + w0, n * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, stream_), "rank upload");
+  dirty_word_ = SIZE_MAX;
+  return SVDB_OK;
+}
+
+uint32_t Engine::select_slot(size_t logical) const {
+  if (!has_deletes_) return (uint32_t)logical;
+  const size_t nw = (nphys_ + 63) / 64;
+  size_t w = (size_t)(std::upper_bound(h_rank_.begin(), h_rank_.begin() + nw, (uint32_t)logical) -
+                      h_rank_.begin()) - 1;
+  uint64_t word = h_alive_[w];
+  size_t j = logical - h_rank_[w];
+  for (size_t t = 0; t < j; t++) word &= word - 1;
+  return (uint32_t)(w * 64 + __builtin_ctzll(word));
+}
+
+int Engine::slot_of(size_t logical, uint32_t* slot) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (logical >= nalive_) {
+    set_error(SVDB_EINVAL, "index out of range");
+    return SVDB_EINVAL;
+  }
+  *slot = select_slot(logical);
+  return SVDB_OK;
+}
+
+int Engine::append(const double* rows, size_t n, size_t lds, size_t dim, uint32_t* first_slot) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!ok_) return SVDB_EDEVICE;
+  if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  int rc = ensure_dim(dim);
+  if (rc) return rc;
+  if ((rc = ensure_capacity(nphys_ + n))) return rc;
+  const size_t s0 = nphys_;
+  if (n) {
+    if (ld_ != d_)
+      HIPCHK(hipMemsetAsync(X_ + s0 * ld_, 0, n * ld_ * sizeof(double), stream_), "memset");
+    HIPCHK(hipMemcpy2DAsync(X_ + s0 * ld_, ld_ * sizeof(double), rows, lds * sizeof(double),
+                            d_ * sizeof(double), n, hipMemcpyHostToDevice, stream_), "row upload");
+    HIPCHK(launch_row_norms(X_, ld_, nullptr, (uint32_t)s0, (uint32_t)n, (uint32_t)d_, nx2_, xn_,
+                            stream_), "k_row_norms");
+  }
+  for (size_t s = s0; s < s0 + n; s++) {
+    if ((s & 63) == 0) h_rank_[s >> 6] = (uint32_t)nalive_;
+    h_alive_[s >> 6] |= 1ull << (s & 63);
+    nalive_++;
+  }
+  dirty_word_ = std::min(dirty_word_, s0 / 64);
+  nphys_ += n;
+  HIPCHK(hipStreamSynchronize(stream_), "append");
+  if (first_slot) *first_slot = (uint32_t)s0;
+  return SVDB_OK;
+}
+
+int Engine::overwrite(size_t logical, const double* row, size_t dim) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!ok_) return SVDB_EDEVICE;
+  if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  if (logical >= nalive_) {
+    set_error(SVDB_EINVAL, "index out of range");
+    return SVDB_EINVAL;
+  }
+  if (dim != d_) {
+    set_error(SVDB_EINVAL, "vector dimension differs from the store's");
+    return SVDB_EINVAL;
+  }
+  const uint32_t s = select_slot(logical);
+  HIPCHK(hipMemcpyAsync(X_ + (size_t)s * ld_, row, d_ * sizeof(double), hipMemcpyHostToDevice,
+                        stream_), "row upload");
+  HIPCHK(launch_row_norms(X_, ld_, nullptr, s, 1, (uint32_t)d_, nx2_, xn_, stream_), "k_row_norms");
+  HIPCHK(hipStreamSynchronize(stream_), "overwrite");
+  return SVDB_OK;
+}
+
+int Engine::erase(size_t logical) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!ok_) return SVDB_EDEVICE;
+  if (logical >= nalive_) {
+    set_error(SVDB_EINVAL, "index out of range");
+    return SVDB_EINVAL;
+  }
+  const uint32_t s = select_slot(logical);
+  const size_t w = s >> 6, nw = (nphys_ + 63) / 64;
+  h_alive_[w] &= ~(1ull << (s & 63));
+  for (size_t v = w + 1; v < nw; v++) h_rank_[v]--;
+  nalive_--;
+  has_deletes_ = true;
+  dirty_word_ = std::min(dirty_word_, w);
+  return SVDB_OK;
+}
+
+// ------------------------------------------------------------------ profiling
+
+hipEvent_t Engine::take_event() {
+  if (!ev_pool_.empty()) {
+    hipEvent_t e = ev_pool_.back();
+    ev_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void Engine::prof_begin(int which) {
+  if (!prof_) return;
+  ProfSlot ps{which, take_event(), take_event()};
+  (void)hipEventRecord(ps.a, stream_);
+  prof_pending_.push_back(ps);
+}
+
+void Engine::prof_end(int which) {
+  if (!prof_) return;
+  for (auto it = prof_pending_.rbegin(); it != prof_pending_.rend(); ++it)
+    if (it->kind == which) {
+      (void)hipEventRecord(it->b, stream_);
+      return;
+    }
+}
+
+void Engine::prof_collect() {
+  for (auto& ps : prof_pending_) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ps.a, ps.b) == hipSuccess) {
+      kt_[ps.kind].launches++;
+      kt_[ps.kind].total_ms += ms;
+    }
+    ev_pool_.push_back(ps.a);
+    ev_pool_.push_back(ps.b);
+  }
+  prof_pending_.clear();
+}
+
+int Engine::set_profiling(bool on) {
+  std::lock_guard<std::mutex> g(mu_);
+  prof_ = on;
+  for (auto& k : kt_) k = KernelTime{};
+  return SVDB_OK;
+}
+
+void Engine::kernel_times(KernelTime* out) {
+  std::lock_guard<std::mutex> g(mu_);
+  for (int i = 0; i < SVDB_K_COUNT; i++) out[i] = kt_[i];
+}
+
+EngineStats Engine::stats() {
+  std::lock_guard<std::mutex> g(mu_);
+  return st_;
+}
+
+int Engine::set_budget(size_t bytes) {
+  std::lock_guard<std::mutex> g(mu_);
+  budget_ = bytes ? bytes : SIZE_MAX;
+  return SVDB_OK;
+}
+
+int Engine::set_shortlist(size_t L) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (L < 1 || L > SVDB_MAX_K * 2) {
+    set_error(SVDB_EINVAL, "shortlist width out of range");
+    return SVDB_EINVAL;
+  }
+  shortlist_ = L;
+  return SVDB_OK;
+}
+
+// ------------------------------------------------------------------ search
+
+// Queries per launch of the scan: bounds the partial-list scratch.
+static constexpr size_t kMaxBatch = 8192;
+
+int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk,
+                      int metric, int kmax, uint64_t* dids, double* dscores, uint32_t ostride) {
+  int rc;
+  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
+  uint32_t nslices = 0, rps = 0;
+  if (nphys_ > 0) {
+    const size_t groups = (nq + kScanTQ - 1) / kScanTQ;
+    size_t want = (1024 + groups - 1) / groups;
+    size_t maxs = std::max<size_t>(1, nphys_ / 1024);
+    nslices = (uint32_t)std::max<size_t>(1, std::min(want, maxs));
+    rps = (uint32_t)((nphys_ + nslices - 1) / nslices);
+  }
+  const uint32_t P = nslices * kWavesPerBlock;
+  const size_t nlist = (size_t)nq * P * kmax;
+  if ((rc = scratch(&s_pk_, &s_pk_n_, std::max<size_t>(nlist, 1) * sizeof(uint64_t)))) return rc;
+  if ((rc = scratch(&s_ps_, &s_ps_n_, std::max<size_t>(nlist, 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&s_qn_, &s_qn_n_, (size_t)nq * sizeof(double)))) return rc;
+  if ((rc = sync_alive())) return rc;
+  if (metric == kCos) HIPCHK(launch_query_norms(dQ, ldq, nq, D, (double*)s_qn_, stream_), "k_query_norms");
+  if (nslices) {
+    ScanF64Args a{sv, dQ, ldq, (const double*)s_qn_, dk, nq, D, rps, P, (uint32_t)kmax,
+                  (uint64_t*)s_pk_, (uint32_t*)s_ps_};
+    prof_begin(SVDB_K_SCAN_FP64);
+    HIPCHK(launch_scan_f64(a, metric, kmax, nslices, stream_), "k_scan_f64");
+    prof_end(SVDB_K_SCAN_FP64);
+  }
+  MergeArgs m{sv, dk, nq, P, (uint32_t)kmax, (const uint64_t*)s_pk_, (const uint32_t*)s_ps_,
+              ostride, dids, dscores};
+  prof_begin(SVDB_K_MERGE);
+  HIPCHK(launch_merge(m, metric, kmax, stream_), "k_merge");
+  prof_end(SVDB_K_MERGE);
+  return SVDB_OK;
+}
+
+static int check_search(size_t d, size_t D, int metric, int precision) {
+  if (metric < 0 || metric > 2) {
+    set_error(SVDB_EINVAL, "unknown metric");
+    return SVDB_EINVAL;
+  }
+  if (precision < SVDB_PREC_FP64 || precision > SVDB_PREC_AUTO) {
+    set_error(SVDB_EINVAL, "unknown precision");
+    return SVDB_EINVAL;
+  }
+  if (D == 0 || D > d) {
+    set_error(SVDB_EINVAL, "compared columns must be in [1, store dimension]");
+    return SVDB_EINVAL;
+  }
+  if (metric != kL2 && D != d) {
+    set_error(SVDB_EINVAL, "cosine and dot compare all columns");
+    return SVDB_EINVAL;
+  }
+  return SVDB_OK;
+}
+
+int Engine::search_device(const double* dQ, size_t nq, size_t ldq, size_t D, size_t k,
+                          int metric, int precision, uint64_t* dids, double* dscores) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!ok_) return SVDB_EDEVICE;
+  if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  if (d_ == 0) {
+    set_error(SVDB_EEMPTY, "store has no rows");
+    return SVDB_EEMPTY;
+  }
+  int rc = check_search(d_, D, metric, precision);
+  if (rc) return rc;
+  if (k < 1 || k > SVDB_MAX_K || ldq < D) {
+    set_error(SVDB_EINVAL, "k must be in [1, 128] and ldq >= compared columns");
+    return SVDB_EINVAL;
+  }
+  const size_t nb = std::min(nq, kMaxBatch);
+  if ((rc = scratch(&s_k_, &s_k_n_, nb * sizeof(int)))) return rc;
+  std::vector<int> hk(nb, (int)k);
+  HIPCHK(hipMemcpyAsync(s_k_, hk.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
+  for (size_t q0 = 0; q0 < nq; q0 += kMaxBatch) {
+    const uint32_t n = (uint32_t)std::min(kMaxBatch, nq - q0);
+    rc = run_exact(dQ + q0 * ldq, ldq, n, (uint32_t)D, (const int*)s_k_, metric, (int)k,
+                   dids + q0 * k, dscores + q0 * k, (uint32_t)k);
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(stream_), "search");
+  prof_collect();
+  st_.searches++;
+  st_.queries += nq;
+  return SVDB_OK;
+}
+
+int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const size_t* k,
+                        const int* metric, int precision, size_t ostride, size_t* ids,
+                        double* scores) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!ok_) return SVDB_EDEVICE;
+  if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  if (nq == 0) return SVDB_OK;
+  size_t kmax_all = 0;
+  for (size_t i = 0; i < nq; i++) {
+    if (k[i] < 1 || k[i] > SVDB_MAX_K) {
+      set_error(SVDB_EINVAL, "k must be in [1, 128]");
+      return SVDB_EINVAL;
+    }
+    kmax_all = std::max(kmax_all, k[i]);
+  }
+  if (ostride < kmax_all) {
+    set_error(SVDB_EINVAL, "out_stride smaller than the largest k");
+    return SVDB_EINVAL;
+  }
+  if (d_ == 0) {
+    // Empty store: nothing to rank.
+    for (size_t i = 0; i < nq; i++)
+      for (size_t j = 0; j < k[i]; j++) {
+        ids[i * ostride + j] = (size_t)-1;
+        scores[i * ostride + j] = NAN;
+      }
+    return SVDB_OK;
+  }
+  int rc;
+  for (size_t i = 0; i < nq; i++)
+    if ((rc = check_search(d_, D, metric[i], precision))) return rc;
+
+  std::vector<double> hq;
+  std::vector<int> hk;
+  std::vector<uint64_t> hid;
+  std::vector<double> hsc;
+  std::vector<size_t> sel;
+  for (int m = 0; m < 3; m++) {
+    sel.clear();
+    for (size_t i = 0; i < nq; i++)
+      if (metric[i] == m) sel.push_back(i);
+    for (size_t b0 = 0; b0 < sel.size(); b0 += kMaxBatch) {
+      const size_t n = std::min(kMaxBatch, sel.size() - b0);
+      size_t kmax = 0;
+      hq.assign(n * D, 0.0);
+      hk.resize(n);
+      for (size_t t = 0; t < n; t++) {
+        const size_t i = sel[b0 + t];
+        std::memcpy(&hq[t * D], Q + i * ldq, D * sizeof(double));
+        hk[t] = (int)k[i];
+        kmax = std::max(kmax, k[i]);
+      }
+      if ((rc = scratch(&s_q_, &s_q_n_, n * D * sizeof(double)))) return rc;
+      if ((rc = scratch(&s_k_, &s_k_n_, n * sizeof(int)))) return rc;
+      if ((rc = scratch(&s_ids_, &s_ids_n_, n * kmax * sizeof(uint64_t)))) return rc;
+      if ((rc = scratch(&s_sc_, &s_sc_n_, n * kmax * sizeof(double)))) return rc;
+      HIPCHK(hipMemcpyAsync(s_q_, hq.data(), n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
+      HIPCHK(hipMemcpyAsync(s_k_, hk.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
+      rc = run_exact((const double*)s_q_, D, (uint32_t)n, (uint32_t)D, (const int*)s_k_, m,
+                     (int)kmax, (uint64_t*)s_ids_, (double*)s_sc_, (uint32_t)kmax);
+      if (rc) return rc;
+      hid.resize(n * kmax);
+      hsc.resize(n * kmax);
+      HIPCHK(hipMemcpyAsync(hid.data(), s_ids_, n * kmax * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
+      HIPCHK(hipMemcpyAsync(hsc.data(), s_sc_, n * kmax * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
+      HIPCHK(hipStreamSynchronize(stream_), "search");
+      prof_collect();
+      for (size_t t = 0; t < n; t++) {
+        const size_t i = sel[b0 + t];
+        for (size_t j = 0; j < k[i]; j++) {
+          ids[i * ostride + j] = (size_t)hid[t * kmax + j];
+          scores[i * ostride + j] = hsc[t * kmax + j];
+        }
+      }
+    }
+  }
+  st_.searches++;
+  st_.queries += nq;
+  return SVDB_OK;
+}
+
+// ------------------------------------------------------------------ pairs
+
+int Engine::pairs_f64(const size_t* a, const size_t* b, size_t np, int metric, double* out) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!ok_) return SVDB_EDEVICE;
+  if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  if (metric < 0 || metric > 2) {
+    set_error(SVDB_EINVAL, "unknown metric");
+    return SVDB_EINVAL;
+  }
+  if (np == 0) return SVDB_OK;
+  std::vector<uint32_t> hs(2 * np);
+  for (size_t p = 0; p < np; p++) {
+    if (a[p] >= nalive_ || b[p] >= nalive_) {
+      set_error(SVDB_EINVAL, "index out of range");
+      return SVDB_EINVAL;
+    }
+    hs[p] = select_slot(a[p]);
+    hs[np + p] = select_slot(b[p]);
+  }
+  int rc;
+  if ((rc = scratch(&s_pair_, &s_pair_n_, 2 * np * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&s_pout_, &s_pout_n_, np * sizeof(double)))) return rc;
+  HIPCHK(hipMemcpyAsync(s_pair_, hs.data(), 2 * np * sizeof(uint32_t), hipMemcpyHostToDevice, stream_), "pairs upload");
+  const uint32_t* sa = (const uint32_t*)s_pair_;
+  HIPCHK(launch_pair_f64(X_, ld_, sa, sa + np, (uint32_t)np, (uint32_t)d_, metric,
+                         (double*)s_pout_, stream_), "k_pair_f64");
+  HIPCHK(hipMemcpyAsync(out, s_pout_, np * sizeof(double), hipMemcpyDeviceToHost, stream_), "pairs download");
+  HIPCHK(hipStreamSynchronize(stream_), "pairs");
+  return SVDB_OK;
+}
+
+}  // namespace svdb

@@ -1,0 +1,151 @@
+// svdb_engine.h — device store + search engine behind the C ABI (internal).
+//
+// One Engine per VectorDatabase (and per stand-alone KDTree).  It owns, in
+// HBM:
+//   X       FP64 master rows, slot-major, row stride ld = round_up(d, 8)
+//   nx2, xn FP64 Σ fma(x,x) and its sqrt per slot
+//   alive   1 bit per slot; rank_base[w] = alive slots before word w
+// Slots are append-only: an insert takes the next slot, an update rewrites
+// its slot in place, a delete clears the slot's alive bit (a tombstone).
+// Slot order therefore equals logical (positional) order, so the lowest
+// slot on a tie is the lowest logical index, and a logical index is the
+// rank of its slot among the alive slots.  Compaction squeezes tombstones
+// out when they pass a quarter of the slots.
+//
+// Every public method takes the engine mutex: callers may enter from any
+// thread (the reference server runs one thread per connection).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "svdb.h"
+#include "svdb_kernels.h"
+
+namespace svdb {
+
+void set_error(int code, const std::string& msg);
+const char* last_error();
+
+// The reference's float arithmetic for one pair of host vectors of length n
+// (vector_database.c:301-352), computed on the GPU: both vectors are staged
+// into device memory and k_pair_ref32 runs on a process-wide stream.
+int pair_ref32_host(const double* a, const double* b, size_t n, int metric, float* out);
+
+// Device used by every engine in this process (SVDB_DEVICE, default 0).
+int device_init();
+
+struct KernelTime {
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+};
+
+struct EngineStats {
+  uint64_t searches = 0, queries = 0, certified = 0, fallbacks = 0;
+  double last_max_rel_err = 0.0;
+};
+
+class Engine {
+ public:
+  explicit Engine(size_t initial_slots);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  bool ok() const { return ok_; }
+  size_t dim() const { return d_; }
+  size_t rows() const { return nalive_; }
+  size_t phys_rows() const { return nphys_; }
+  size_t device_bytes() const { return dev_bytes_; }
+  size_t budget() const { return budget_; }
+
+  // ---- store (host rows; row i at rows + i*lds) ----
+  int append(const double* rows, size_t n, size_t lds, size_t dim, uint32_t* first_slot);
+  int overwrite(size_t logical, const double* row, size_t dim);
+  int erase(size_t logical);
+  int slot_of(size_t logical, uint32_t* slot);
+
+  // ---- search ----
+  // Host queries; per-query k and metric; ids/scores row stride ostride.
+  int search_host(const double* Q, size_t nq, size_t ldq, size_t D, const size_t* k,
+                  const int* metric, int precision, size_t ostride, size_t* ids, double* scores);
+  // Device queries and outputs; one k and metric.
+  int search_device(const double* dQ, size_t nq, size_t ldq, size_t D, size_t k, int metric,
+                    int precision, uint64_t* dids, double* dscores);
+
+  // ---- pair metrics over stored rows (logical indices) ----
+  int pairs_f64(const size_t* a, const size_t* b, size_t np, int metric, double* out);
+
+  // ---- knobs / stats ----
+  int set_budget(size_t bytes);
+  int set_shortlist(size_t L);
+  int set_profiling(bool on);
+  void kernel_times(KernelTime* out);
+  EngineStats stats();
+
+  std::mutex& mutex() { return mu_; }
+
+ private:
+  int ensure_dim(size_t d);
+  int ensure_capacity(size_t slots);
+  int sync_alive();
+  int dev_alloc(void** p, size_t bytes);
+  void dev_free(void* p, size_t bytes);
+  int scratch(void** p, size_t* have, size_t need);
+  int run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk, int metric,
+                int kmax, uint64_t* dids, double* dscores, uint32_t ostride);
+  uint32_t select_slot(size_t logical) const;
+  void prof_begin(int which);
+  void prof_end(int which);
+  void prof_collect();
+  hipEvent_t take_event();
+
+  bool ok_ = false;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::mutex mu_;
+
+  size_t d_ = 0, ld_ = 0;
+  size_t cap_ = 0, nphys_ = 0, nalive_ = 0, init_slots_ = 0;
+  double* X_ = nullptr;
+  double* nx2_ = nullptr;
+  double* xn_ = nullptr;
+  uint64_t* alive_ = nullptr;
+  uint32_t* rank_ = nullptr;
+  std::vector<uint64_t> h_alive_;
+  std::vector<uint32_t> h_rank_;
+  bool has_deletes_ = false;
+  size_t dirty_word_ = SIZE_MAX;  // first bitmap word not yet on the device
+
+  size_t dev_bytes_ = 0;
+  size_t budget_ = SIZE_MAX;
+  size_t shortlist_ = 64;
+
+  // scratch (grow-only)
+  void* s_q_ = nullptr;      size_t s_q_n_ = 0;
+  void* s_qn_ = nullptr;     size_t s_qn_n_ = 0;
+  void* s_k_ = nullptr;      size_t s_k_n_ = 0;
+  void* s_pk_ = nullptr;     size_t s_pk_n_ = 0;
+  void* s_ps_ = nullptr;     size_t s_ps_n_ = 0;
+  void* s_ids_ = nullptr;    size_t s_ids_n_ = 0;
+  void* s_sc_ = nullptr;     size_t s_sc_n_ = 0;
+  void* s_pair_ = nullptr;   size_t s_pair_n_ = 0;
+  void* s_pout_ = nullptr;   size_t s_pout_n_ = 0;
+
+  struct ProfSlot {
+    int kind;
+    hipEvent_t a, b;
+  };
+  bool prof_ = false;
+  std::vector<ProfSlot> prof_pending_;
+  std::vector<hipEvent_t> ev_pool_;
+  KernelTime kt_[SVDB_K_COUNT];
+  EngineStats st_;
+};
+
+}  // namespace svdb

@@ -1,0 +1,69 @@
+// svdb_kernels.h — launch interface of the gfx950 kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace svdb {
+
+enum Metric { kL2 = 0, kCos = 1, kDot = 2 };
+
+// Queries per wavefront in the FP64 scan; queries are wave-uniform there.
+constexpr int kScanTQ = 16;
+// Rows per wavefront step in the FP64 scan (two per lane).
+constexpr int kScanRows = 128;
+// Wavefronts per block in the FP64 scan and merge kernels.
+constexpr int kWavesPerBlock = 4;
+
+struct StoreView {
+  const double* X;          // FP64 master rows, row stride ldx (elements)
+  size_t ldx;
+  const double* xn;         // sqrt(Σ fma(x_i,x_i,·)) per row
+  const uint64_t* alive;    // 1 bit per slot
+  const uint32_t* rank_base;// alive slots before each 64-slot word
+  uint32_t nrows;           // slots in use
+  int has_deletes;          // 0: logical id == slot
+};
+
+struct ScanF64Args {
+  StoreView st;
+  const double* Q;          // queries, stride ldq
+  size_t ldq;
+  const double* qn;         // per-query norm (cosine)
+  const int* k;             // per-query k
+  uint32_t nq;
+  uint32_t D;               // leading columns compared
+  uint32_t rows_per_slice;
+  uint32_t P;               // partial lists per query
+  uint32_t kstride;         // entries per partial list (>= max k)
+  uint64_t* part_key;       // [nq][P][kstride]
+  uint32_t* part_slot;
+};
+
+struct MergeArgs {
+  StoreView st;
+  const int* k;
+  uint32_t nq;
+  uint32_t P;
+  uint32_t kstride;
+  const uint64_t* part_key;
+  const uint32_t* part_slot;
+  uint32_t ostride;
+  uint64_t* ids;            // logical ids, (uint64)-1 when missing
+  double* scores;           // FP64 scores, NaN when missing
+};
+
+hipError_t launch_row_norms(const double* X, size_t ldx, const uint32_t* slots, uint32_t first,
+                            uint32_t n, uint32_t d, double* nx2, double* xn, hipStream_t s);
+hipError_t launch_query_norms(const double* Q, size_t ldq, uint32_t nq, uint32_t D, double* qn,
+                              hipStream_t s);
+hipError_t launch_scan_f64(const ScanF64Args& a, int metric, int kmax, uint32_t nslices,
+                           hipStream_t s);
+hipError_t launch_merge(const MergeArgs& a, int metric, int kmax, hipStream_t s);
+hipError_t launch_pair_f64(const double* X, size_t ldx, const uint32_t* sa, const uint32_t* sb,
+                           uint32_t np, uint32_t D, int metric, double* out, hipStream_t s);
+hipError_t launch_pair_ref32(const double* M, size_t ld, const uint32_t* sa, const uint32_t* sb,
+                             uint32_t np, uint32_t n, int metric, float* out, hipStream_t s);
+
+}  // namespace svdb

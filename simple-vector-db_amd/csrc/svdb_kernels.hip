@@ -1,0 +1,522 @@
+// svdb_kernels.hip — gfx950 kernels of the similarity / nearest-vector path.
+//
+// Built with -ffp-contract=off: every a*b+c below that is not an explicit
+// fma() is a rounded multiply then a rounded add; the explicit fma() calls
+// are the canonical FP64 steps (correctly rounded v_fma_f64).  sqrt and '/'
+// on doubles lower to the correctly rounded sequences (v_rsq_f64 + Newton
+// steps + fixup; v_div_scale/v_div_fmas/v_div_fixup).
+//
+// Canonical FP64 arithmetic (matches oracle/svdb_oracle.c bit for bit):
+//   L2 key  s = Σ_i fma(t, t, s), t = x_i - q_i, i = 0..D-1 in order
+//   dot key s = Σ_i fma(q_i, x_i, s)
+//   cosine  dot / (sqrt(Σ fma(q,q)) * sqrt(Σ fma(x,x)))
+// Ranking: ascending "rank key" (L2: s; dot/cos: -score) mapped to a uint64
+// that orders like the doubles (-0 folded to +0, NaN above +inf), then the
+// slot (physical row), which orders like the logical index.
+#include "svdb_kernels.h"
+
+#include <math.h>
+
+namespace svdb {
+
+// ----------------------------------------------------------------- helpers
+
+__device__ __forceinline__ uint64_t rank_bits(double v) {
+  v = v + 0.0;  // -0 → +0 (not folded: strict FP)
+  uint64_t u = (uint64_t)__double_as_longlong(v);
+  if (v != v) u = 0x7FF8000000000000ull;
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double rank_value(uint64_t u) {
+  uint64_t b = (u >> 63) ? (u & 0x7FFFFFFFFFFFFFFFull) : ~u;
+  return __longlong_as_double((long long)b);
+}
+
+__device__ __forceinline__ bool lex_lt(uint64_t ka, uint32_t sa, uint64_t kb, uint32_t sb) {
+  return (ka < kb) | ((ka == kb) & (sa < sb));
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t uniform32(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// A sorted list of (key, slot) spread over a wavefront: entry e lives in
+// lane e % 64, register e / 64.  Unused entries hold the sentinel
+// (UINT64_MAX, UINT32_MAX), which is above every real entry (slots are
+// < UINT32_MAX).
+template <int KPL>
+struct WList {
+  uint64_t k[KPL];
+  uint32_t s[KPL];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      k[p] = ~0ull;
+      s[p] = ~0u;
+    }
+  }
+  // Entry kk-1 (the current k-th best), wave-uniform.
+  __device__ __forceinline__ void kth(int kk, uint64_t& tk, uint32_t& ts) const {
+    int e = kk - 1, p = e >> 6, ln = e & 63;
+    uint64_t a = k[0];
+    uint32_t b = s[0];
+#pragma unroll
+    for (int q = 1; q < KPL; q++)
+      if (p == q) {
+        a = k[q];
+        b = s[q];
+      }
+    tk = uniform64(__shfl(a, ln));
+    ts = uniform32(__shfl(b, ln));
+  }
+  // Insert (K, S) (wave-uniform) keeping the first kk entries the kk best.
+  __device__ __forceinline__ void insert(uint64_t K, uint32_t S, int kk, int lane) {
+    int pos = 0;
+#pragma unroll
+    for (int p = 0; p < KPL; p++) pos += __popcll(__ballot(lex_lt(k[p], s[p], K, S)));
+    if (pos >= kk) return;
+    uint64_t pk[KPL];
+    uint32_t ps[KPL];
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      pk[p] = __shfl_up(k[p], 1);
+      ps[p] = __shfl_up(s[p], 1);
+    }
+#pragma unroll
+    for (int p = 1; p < KPL; p++) {
+      uint64_t ck = __shfl(k[p - 1], 63);
+      uint32_t cs = __shfl(s[p - 1], 63);
+      if (lane == 0) {
+        pk[p] = ck;
+        ps[p] = cs;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      int e = p * 64 + lane;
+      if (e > pos) {
+        k[p] = pk[p];
+        s[p] = ps[p];
+      } else if (e == pos) {
+        k[p] = K;
+        s[p] = S;
+      }
+    }
+  }
+};
+
+__device__ __forceinline__ bool slot_alive(const uint64_t* alive, uint32_t r) {
+  return (alive[r >> 6] >> (r & 63)) & 1ull;
+}
+
+__device__ __forceinline__ uint64_t logical_id(const StoreView& st, uint32_t slot) {
+  if (!st.has_deletes) return slot;
+  uint32_t w = slot >> 6;
+  uint64_t below = st.alive[w] & ((1ull << (slot & 63)) - 1ull);
+  return (uint64_t)st.rank_base[w] + (uint64_t)__popcll(below);
+}
+
+// ------------------------------------------------------------ store writes
+
+// One thread per row: the sequential fma chain of the squared norm of a row
+// already copied into the master (padding columns are zero).  Rows are the
+// slots first..first+n-1, or slots[0..n) when a slot list is given.
+__global__ void k_row_norms(const double* __restrict__ X, size_t ldx,
+                            const uint32_t* __restrict__ slots, uint32_t first, uint32_t n,
+                            uint32_t d, double* __restrict__ nx2, double* __restrict__ xn) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t slot = slots ? slots[r] : first + r;
+  const double* x = X + (size_t)slot * ldx;
+  double acc = 0.0;
+  for (uint32_t i = 0; i < d; i++) acc = fma(x[i], x[i], acc);
+  nx2[slot] = acc;
+  xn[slot] = sqrt(acc);
+}
+
+hipError_t launch_row_norms(const double* X, size_t ldx, const uint32_t* slots, uint32_t first,
+                            uint32_t n, uint32_t d, double* nx2, double* xn, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_norms, dim3((n + 255) / 256), dim3(256), 0, s, X, ldx, slots, first, n,
+                     d, nx2, xn);
+  return hipGetLastError();
+}
+
+__global__ void k_query_norms(const double* __restrict__ Q, size_t ldq, uint32_t nq, uint32_t D,
+                              double* __restrict__ qn) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const double* q = Q + (size_t)i * ldq;
+  double acc = 0.0;
+  for (uint32_t c = 0; c < D; c++) acc = fma(q[c], q[c], acc);
+  qn[i] = sqrt(acc);
+}
+
+hipError_t launch_query_norms(const double* Q, size_t ldq, uint32_t nq, uint32_t D, double* qn,
+                              hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_query_norms, dim3((nq + 255) / 256), dim3(256), 0, s, Q, ldq, nq, D, qn);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------- FP64 scan
+//
+// Grid (query groups of kScanTQ, row slices); 4 wavefronts per block share
+// the block's queries and take interleaved 128-row steps of the slice.  Each
+// lane owns two rows per step (row base+lane and base+64+lane) and keeps the
+// kScanTQ×2 FP64 accumulators; query values are wave-uniform loads.  After
+// a step every (query, row) key is tested against that query's current k-th
+// best (wave-uniform); hits are inserted into the query's wave-distributed
+// sorted list.  Each wavefront writes its lists as one partial list per
+// query.
+
+template <int METRIC>
+__device__ __forceinline__ void accum8(double (&acc)[kScanTQ][2], const double (&x0)[8],
+                                       const double (&x1)[8], const double* __restrict__ Qg,
+                                       size_t ldq, uint32_t c) {
+  // Query-outer order keeps one query's 8 values live in scalar registers;
+  // each (query, row) chain still advances in element order.
+#pragma unroll
+  for (int j = 0; j < kScanTQ; j++) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      double q = Qg[(size_t)j * ldq + c + i];
+      if (METRIC == kL2) {
+        double t0 = x0[i] - q, t1 = x1[i] - q;
+        acc[j][0] = fma(t0, t0, acc[j][0]);
+        acc[j][1] = fma(t1, t1, acc[j][1]);
+      } else {
+        acc[j][0] = fma(q, x0[i], acc[j][0]);
+        acc[j][1] = fma(q, x1[i], acc[j][1]);
+      }
+    }
+  }
+}
+
+template <int METRIC, int KPL>
+__global__ __launch_bounds__(256) void k_scan_f64(ScanF64Args a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint32_t q0 = blockIdx.x * kScanTQ;
+  const uint32_t slice = blockIdx.y;
+  const uint32_t r_begin = slice * a.rows_per_slice;
+  const uint32_t r_end = min(r_begin + a.rows_per_slice, a.st.nrows);
+  // Queries past nq reuse the last query's row (their lists are not written).
+  const uint32_t qlast = a.nq - 1;
+  const double* __restrict__ Qg = a.Q + (size_t)min(q0, qlast) * a.ldq;
+  const bool full_group = q0 + kScanTQ <= a.nq;
+
+  WList<KPL> L[kScanTQ];
+  uint64_t tk[kScanTQ];
+  uint32_t ts[kScanTQ];
+  int kq[kScanTQ];
+  double qn[kScanTQ];
+#pragma unroll
+  for (int j = 0; j < kScanTQ; j++) {
+    L[j].init();
+    tk[j] = ~0ull;
+    ts[j] = ~0u;
+    uint32_t qi = min(q0 + j, qlast);
+    kq[j] = a.k[qi];
+    qn[j] = METRIC == kCos ? a.qn[qi] : 0.0;
+  }
+
+  for (uint32_t base = r_begin + wave * kScanRows; base < r_end;
+       base += kWavesPerBlock * kScanRows) {
+    const uint32_t r0 = base + lane, r1 = base + 64 + lane;
+    const uint32_t c0 = min(r0, a.st.nrows - 1), c1 = min(r1, a.st.nrows - 1);
+    const double* __restrict__ x0p = a.st.X + (size_t)c0 * a.st.ldx;
+    const double* __restrict__ x1p = a.st.X + (size_t)c1 * a.st.ldx;
+    double acc[kScanTQ][2];
+#pragma unroll
+    for (int j = 0; j < kScanTQ; j++) acc[j][0] = acc[j][1] = 0.0;
+
+    uint32_t c = 0;
+    if (full_group) {
+      for (; c + 8 <= a.D; c += 8) {
+        double x0[8], x1[8];
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+          double2 v0 = *reinterpret_cast<const double2*>(x0p + c + i);
+          double2 v1 = *reinterpret_cast<const double2*>(x1p + c + i);
+          x0[i] = v0.x; x0[i + 1] = v0.y;
+          x1[i] = v1.x; x1[i + 1] = v1.y;
+        }
+        accum8<METRIC>(acc, x0, x1, Qg, a.ldq, c);
+      }
+    }
+    for (; c < a.D; c++) {
+      double xa = x0p[c], xb = x1p[c];
+#pragma unroll
+      for (int j = 0; j < kScanTQ; j++) {
+        double q = a.Q[(size_t)min(q0 + j, qlast) * a.ldq + c];
+        if (METRIC == kL2) {
+          double t0 = xa - q, t1 = xb - q;
+          acc[j][0] = fma(t0, t0, acc[j][0]);
+          acc[j][1] = fma(t1, t1, acc[j][1]);
+        } else {
+          acc[j][0] = fma(q, xa, acc[j][0]);
+          acc[j][1] = fma(q, xb, acc[j][1]);
+        }
+      }
+    }
+
+    const bool v0 = r0 < r_end && slot_alive(a.st.alive, c0);
+    const bool v1 = r1 < r_end && slot_alive(a.st.alive, c1);
+    double xn0 = 0.0, xn1 = 0.0;
+    if (METRIC == kCos) {
+      xn0 = a.st.xn[c0];
+      xn1 = a.st.xn[c1];
+    }
+#pragma unroll
+    for (int j = 0; j < kScanTQ; j++) {
+      double s0 = acc[j][0], s1 = acc[j][1];
+      if (METRIC == kDot) {
+        s0 = -s0;
+        s1 = -s1;
+      } else if (METRIC == kCos) {
+        s0 = -(s0 / (qn[j] * xn0));
+        s1 = -(s1 / (qn[j] * xn1));
+      }
+      const uint64_t k0 = rank_bits(s0), k1 = rank_bits(s1);
+      uint64_t m0 = __ballot(v0 && lex_lt(k0, r0, tk[j], ts[j]));
+      uint64_t m1 = __ballot(v1 && lex_lt(k1, r1, tk[j], ts[j]));
+      if (m0 | m1) {
+        while (m0) {
+          int src = __ffsll((long long)m0) - 1;
+          m0 &= m0 - 1;
+          L[j].insert(uniform64(__shfl(k0, src)), base + src, kq[j], lane);
+        }
+        while (m1) {
+          int src = __ffsll((long long)m1) - 1;
+          m1 &= m1 - 1;
+          L[j].insert(uniform64(__shfl(k1, src)), base + 64 + src, kq[j], lane);
+        }
+        L[j].kth(kq[j], tk[j], ts[j]);
+      }
+    }
+  }
+
+  const uint32_t list = slice * kWavesPerBlock + wave;
+#pragma unroll
+  for (int j = 0; j < kScanTQ; j++) {
+    if (q0 + j >= a.nq) break;
+    size_t o = ((size_t)(q0 + j) * a.P + list) * a.kstride;
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      int e = p * 64 + lane;
+      if (e < kq[j]) {
+        a.part_key[o + e] = L[j].k[p];
+        a.part_slot[o + e] = L[j].s[p];
+      }
+    }
+  }
+}
+
+template <int METRIC, int KPL>
+static hipError_t scan_f64_t(const ScanF64Args& a, uint32_t nslices, hipStream_t s) {
+  dim3 grid((a.nq + kScanTQ - 1) / kScanTQ, nslices);
+  hipLaunchKernelGGL((k_scan_f64<METRIC, KPL>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_f64(const ScanF64Args& a, int metric, int kmax, uint32_t nslices,
+                           hipStream_t s) {
+  if (a.nq == 0) return hipSuccess;
+  const bool two = kmax > 64;
+  switch (metric) {
+    case kL2: return two ? scan_f64_t<kL2, 2>(a, nslices, s) : scan_f64_t<kL2, 1>(a, nslices, s);
+    case kCos: return two ? scan_f64_t<kCos, 2>(a, nslices, s) : scan_f64_t<kCos, 1>(a, nslices, s);
+    case kDot: return two ? scan_f64_t<kDot, 2>(a, nslices, s) : scan_f64_t<kDot, 1>(a, nslices, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------- list merge
+//
+// One wavefront per query merges the query's P sorted partial lists into
+// its final top-k, then writes logical ids and FP64 scores.  A partial list
+// whose first entry does not beat the current k-th best is skipped whole.
+
+template <int METRIC, int KPL>
+__global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t q = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (q >= a.nq) return;
+  const int kk = a.k[q];
+  WList<KPL> L;
+  L.init();
+  uint64_t tk = ~0ull;
+  uint32_t ts = ~0u;
+  for (uint32_t p = 0; p < a.P; p++) {
+    const size_t o = ((size_t)q * a.P + p) * a.kstride;
+    const uint64_t e0k = uniform64(a.part_key[o]);
+    const uint32_t e0s = uniform32(a.part_slot[o]);
+    if (!lex_lt(e0k, e0s, tk, ts)) continue;
+#pragma unroll
+    for (int pp = 0; pp < KPL; pp++) {
+      const int e = pp * 64 + lane;
+      uint64_t ck = ~0ull;
+      uint32_t cs = ~0u;
+      if (e < kk) {
+        ck = a.part_key[o + e];
+        cs = a.part_slot[o + e];
+      }
+      uint64_t m = __ballot(lex_lt(ck, cs, tk, ts));
+      while (m) {
+        int src = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        L.insert(uniform64(__shfl(ck, src)), uniform32(__shfl(cs, src)), kk, lane);
+      }
+    }
+    L.kth(kk, tk, ts);
+  }
+#pragma unroll
+  for (int pp = 0; pp < KPL; pp++) {
+    const int e = pp * 64 + lane;
+    if (e < kk) {
+      uint64_t id = ~0ull;
+      double sc = __longlong_as_double(0x7FF8000000000000ll);
+      if (L.s[pp] != ~0u) {
+        double v = rank_value(L.k[pp]);
+        // NaN keys come back as the one canonical quiet NaN (its sign is
+        // not meaningful); a zero dot/cos rank key is a +0 score.
+        if (v == v) sc = METRIC == kL2 ? sqrt(v) : (v == 0.0 ? 0.0 : -v);
+        id = logical_id(a.st, L.s[pp]);
+      }
+      a.ids[(size_t)q * a.ostride + e] = id;
+      a.scores[(size_t)q * a.ostride + e] = sc;
+    }
+  }
+}
+
+template <int METRIC, int KPL>
+static hipError_t merge_t(const MergeArgs& a, hipStream_t s) {
+  dim3 grid((a.nq + kWavesPerBlock - 1) / kWavesPerBlock);
+  hipLaunchKernelGGL((k_merge<METRIC, KPL>), grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge(const MergeArgs& a, int metric, int kmax, hipStream_t s) {
+  if (a.nq == 0) return hipSuccess;
+  const bool two = kmax > 64;
+  switch (metric) {
+    case kL2: return two ? merge_t<kL2, 2>(a, s) : merge_t<kL2, 1>(a, s);
+    case kCos: return two ? merge_t<kCos, 2>(a, s) : merge_t<kCos, 1>(a, s);
+    case kDot: return two ? merge_t<kDot, 2>(a, s) : merge_t<kDot, 1>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------- pair metrics
+//
+// FP64: one wavefront per pair, coalesced loads; lane l accumulates the
+// elements i ≡ l (mod 64) in increasing order with fma, then six xor
+// butterfly steps (offsets 32..1).  Every lane ends with the same sum.
+
+template <int METRIC>
+__global__ __launch_bounds__(256) void k_pair_f64(const double* __restrict__ X, size_t ldx,
+                                                  const uint32_t* __restrict__ sa,
+                                                  const uint32_t* __restrict__ sb, uint32_t np,
+                                                  uint32_t D, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t p = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (p >= np) return;
+  const double* __restrict__ A = X + (size_t)sa[p] * ldx;
+  const double* __restrict__ B = X + (size_t)sb[p] * ldx;
+  double d = 0.0, na = 0.0, nb = 0.0;
+  for (uint32_t i = lane; i < D; i += 64) {
+    double u = A[i], v = B[i];
+    if (METRIC == kL2) {
+      double t = u - v;
+      d = fma(t, t, d);
+    } else {
+      d = fma(u, v, d);
+      if (METRIC == kCos) {
+        na = fma(u, u, na);
+        nb = fma(v, v, nb);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    d = d + __shfl_xor(d, off);
+    if (METRIC == kCos) {
+      na = na + __shfl_xor(na, off);
+      nb = nb + __shfl_xor(nb, off);
+    }
+  }
+  if (lane == 0) {
+    double r = d;
+    if (METRIC == kL2) r = sqrt(d);
+    else if (METRIC == kCos) r = d / (sqrt(na) * sqrt(nb));
+    out[p] = r;
+  }
+}
+
+hipError_t launch_pair_f64(const double* X, size_t ldx, const uint32_t* sa, const uint32_t* sb,
+                           uint32_t np, uint32_t D, int metric, double* out, hipStream_t s) {
+  if (np == 0) return hipSuccess;
+  dim3 grid((np + kWavesPerBlock - 1) / kWavesPerBlock), block(256);
+  switch (metric) {
+    case kL2: hipLaunchKernelGGL(k_pair_f64<kL2>, grid, block, 0, s, X, ldx, sa, sb, np, D, out); break;
+    case kCos: hipLaunchKernelGGL(k_pair_f64<kCos>, grid, block, 0, s, X, ldx, sa, sb, np, D, out); break;
+    case kDot: hipLaunchKernelGGL(k_pair_f64<kDot>, grid, block, 0, s, X, ldx, sa, sb, np, D, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// The reference's own float arithmetic (vector_database.c:301-352), one
+// thread per pair, so the legacy float-returning metrics are bit-identical
+// to the reference: FP64 products added into float accumulators
+// (acc = (float)((double)acc + a*b)); Euclidean squares the float-rounded
+// difference in float; sqrt and the cosine division in double.
+__global__ void k_pair_ref32(const double* __restrict__ M, size_t ld,
+                             const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
+                             uint32_t np, uint32_t n, int metric, float* __restrict__ out) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double* A = M + (size_t)sa[p] * ld;
+  const double* B = M + (size_t)sb[p] * ld;
+  float r;
+  if (metric == kCos) {
+    float dp = 0.0f, na = 0.0f, nb = 0.0f;
+    for (uint32_t i = 0; i < n; i++) {
+      dp = (float)((double)dp + A[i] * B[i]);
+      na = (float)((double)na + A[i] * A[i]);
+      nb = (float)((double)nb + B[i] * B[i]);
+    }
+    r = (float)((double)dp / (sqrt((double)na) * sqrt((double)nb)));
+  } else if (metric == kL2) {
+    float sum = 0.0f;
+    for (uint32_t i = 0; i < n; i++) {
+      float diff = (float)(A[i] - B[i]);
+      float sq = diff * diff;
+      sum = sum + sq;
+    }
+    r = (float)sqrt((double)sum);
+  } else {
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < n; i++) acc = (float)((double)acc + A[i] * B[i]);
+    r = acc;
+  }
+  out[p] = r;
+}
+
+hipError_t launch_pair_ref32(const double* M, size_t ld, const uint32_t* sa, const uint32_t* sb,
+                             uint32_t np, uint32_t n, int metric, float* out, hipStream_t s) {
+  if (np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pair_ref32, dim3((np + 255) / 256), dim3(256), 0, s, M, ld, sa, sb, np, n,
+                     metric, out);
+  return hipGetLastError();
+}
+
+}  // namespace svdb

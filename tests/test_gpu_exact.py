@@ -1,0 +1,279 @@
+"""GPU parity of the exact path, through the C ABI (libsvdb.so).
+
+Bar: ids and FP64 scores bit-identical to the CPU oracle (canonical FP64
+arithmetic, lowest id on ties); float metrics bit-identical to the
+reference's own outputs (tests/golden); find-nearest identical to the
+reference's kd-tree where distances are distinct.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle as O
+import svdb
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    """Bit patterns, with every NaN mapped to one pattern (NaN sign and
+    payload carry no meaning)."""
+    a = np.array(a, dtype=np.float64)
+    a[np.isnan(a)] = np.nan
+    return a.view(np.uint64)
+
+
+def assert_same(ids, sc, rid, rsc, what=""):
+    assert np.array_equal(ids, rid), f"{what}: ids differ at {np.argwhere(ids != rid)[:5]}"
+    assert np.array_equal(bits(sc), bits(rsc)), f"{what}: scores differ"
+
+
+@pytest.fixture(scope="module")
+def c1db(c1_data):
+    X, _, _ = c1_data
+    say("c1db: 10000x128 uniform[-1,1] seed 0, kd dim 3 and 128")
+    dbs = {}
+    for D in (3, 128):
+        db = svdb.Database(kd_dimension=D)
+        db.bulk_insert(X)
+        dbs[D] = db
+    yield dbs
+    for db in dbs.values():
+        db.close()
+
+
+# ---------------------------------------------------------------- C1
+@pytest.mark.parametrize("D,key", [(3, "c1_kd3_ids"), (128, "c1_kd128_ids")])
+def test_c1_find_nearest_equals_reference(c1db, c1_data, golden, D, key):
+    _, Q, _ = c1_data
+    say(f"kdtree_nearest D={D}: 256 queries")
+    got = np.array([c1db[D].kd_nearest(q) for q in Q], dtype=np.uint64)
+    assert np.array_equal(got, golden[key])
+
+
+def test_c1_float_metrics_bit_exact(c1db, c1_data, golden):
+    X, _, P = c1_data
+    say("legacy float metrics: 256 pairs x 3")
+    ref = golden["c1_metric_bits"]
+    for col, m in enumerate((svdb.COS, svdb.L2, svdb.DOT)):
+        got = np.array([svdb.legacy_metric(X[a], X[b], m) for a, b in P], dtype=np.float32)
+        assert np.array_equal(got.view(np.uint32), ref[:, col]), m
+
+
+def test_float_metrics_special_inputs(golden):
+    A, B = golden["spec_a"], golden["spec_b"]
+    ref = golden["spec_metric_bits"]
+    for col, m in enumerate((svdb.COS, svdb.L2, svdb.DOT)):
+        got = np.array([svdb.legacy_metric(a, b, m) for a, b in zip(A, B)], dtype=np.float32)
+        assert np.array_equal(got.view(np.uint32), ref[:, col]), m
+    mm = np.array([svdb.legacy_metric(np.ones(5), np.ones(4), m)
+                   for m in (svdb.COS, svdb.L2, svdb.DOT)], dtype=np.float32)
+    assert np.array_equal(mm.view(np.uint32), golden["mismatch_bits"])
+
+
+# ---------------------------------------------------------------- FP64 scan
+@pytest.fixture(scope="module")
+def gauss():
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((200_000, 128))
+    Q = rng.standard_normal((300, 128))
+    say("gauss store: 200000x128 N(0,1)")
+    db = svdb.Database(kd_dimension=128)
+    db.bulk_insert(X)
+    yield X, Q, db
+    db.close()
+
+
+@pytest.mark.parametrize("metric", ["l2", "cos", "dot"])
+@pytest.mark.parametrize("k", [1, 10, 100])
+def test_fp64_scan_matches_oracle(gauss, metric, k):
+    X, Q, db = gauss
+    say(f"fp64 scan {metric} k={k}: 300 queries over 200000 rows")
+    ids, sc = db.search(Q, k, metric, "fp64")
+    rid, rsc = O.topk(X, Q, k, metric)
+    assert_same(ids, sc, rid, rsc, f"{metric} k={k}")
+
+
+def test_fp64_scan_mixed_batch(gauss):
+    X, Q, db = gauss
+    rng = np.random.default_rng(3)
+    ks = rng.choice([1, 10, 100], size=Q.shape[0])
+    ms = rng.choice(["l2", "cos", "dot"], size=Q.shape[0])
+    say("fp64 scan mixed k/metric batch")
+    ids, sc = db.search(Q, ks, list(ms), "fp64")
+    for i in range(0, Q.shape[0], 37):
+        rid, rsc = O.topk(X, Q[i:i + 1], int(ks[i]), str(ms[i]))
+        assert_same(ids[i:i + 1, :ks[i]], sc[i:i + 1, :ks[i]], rid, rsc, f"query {i}")
+
+
+def test_kd_projection_columns(gauss):
+    X, Q, db = gauss
+    say("L2 over the leading 7 columns (kd dimension 7)")
+    db7 = svdb.Database(kd_dimension=7)
+    db7.bulk_insert(X[:50_000])
+    got = np.array([db7.kd_nearest(q) for q in Q[:40]], dtype=np.uint64)
+    rid, _ = O.topk(X[:50_000], Q[:40], 1, "l2", D=7)
+    assert np.array_equal(got, rid[:, 0])
+    db7.close()
+
+
+# ---------------------------------------------------------------- edge cases
+def test_ties_nan_and_short_store(golden):
+    say("ties / NaN / k > rows")
+    T, TQ = golden["tie_db"], golden["tie_q"]
+    db = svdb.Database(kd_dimension=3)
+    db.bulk_insert(T)
+    ids, sc = db.search(TQ, 3, "l2", "fp64")
+    rid, rsc = O.topk(T, TQ, 3, "l2")
+    assert_same(ids, sc, rid, rsc, "duplicates")
+    assert ids[0, :2].tolist() == [0, 2]
+    db.close()
+    db = svdb.Database(kd_dimension=2)
+    db.bulk_insert(golden["dfs_db"])
+    assert db.kd_nearest(golden["dfs_q"][0]) == 1  # lowest id; the reference answers 2
+    db.close()
+    X = np.array([[1.0, 0.0], [np.nan, 0.0], [0.0, 0.0], [0.5, 0.5]])
+    q = np.array([[1.0, 0.1]])
+    db = svdb.Database(kd_dimension=2)
+    db.bulk_insert(X)
+    for m in ("l2", "cos", "dot"):
+        ids, sc = db.search(q, 6, m, "fp64")
+        rid, rsc = O.topk(X, q, 6, m)
+        assert_same(ids, sc, rid, rsc, m)
+    db.close()
+
+
+def test_empty_store_and_bad_args():
+    db = svdb.Database(kd_dimension=3)
+    assert db.kd_nearest(np.zeros(3)) == 2**64 - 1
+    db.bulk_insert(np.ones((4, 3)))
+    with pytest.raises(svdb.SvdbError):
+        db.search(np.zeros((1, 3)), 129, "l2")
+    with pytest.raises(svdb.SvdbError):
+        db.search(np.zeros((1, 4)), 1, "l2")
+    db.close()
+
+
+# ---------------------------------------------------------------- mutations
+def test_insert_update_delete_then_search():
+    rng = np.random.default_rng(21)
+    rows = [r for r in rng.standard_normal((3000, 64))]
+    db = svdb.Database(kd_dimension=64)
+    say("mutations: 3000 rows, interleaved update/delete/insert")
+    db.bulk_insert(np.array(rows))
+    for step in range(300):
+        op = rng.integers(3)
+        if op == 0:
+            r = rng.standard_normal(64)
+            assert db.insert(r, b"n%d" % step) == len(rows)
+            rows.append(r)
+        elif op == 1:
+            i = int(rng.integers(len(rows)))
+            r = rng.standard_normal(64)
+            db.update(i, r, b"u")
+            rows[i] = r
+        else:
+            i = int(rng.integers(len(rows)))
+            db.delete(i)
+            del rows[i]
+    assert len(db) == len(rows)
+    M = np.array(rows)
+    for i in (0, len(rows) // 2, len(rows) - 1):
+        assert np.array_equal(db.read(i)[1], M[i])
+    Q = rng.standard_normal((40, 64))
+    for m in ("l2", "cos", "dot"):
+        ids, sc = db.search(Q, 10, m, "fp64")
+        rid, rsc = O.topk(M, Q, 10, m)
+        assert_same(ids, sc, rid, rsc, m)
+    got = np.array([db.kd_nearest(q) for q in Q[:10]], dtype=np.uint64)
+    assert np.array_equal(got, O.topk(M, Q[:10], 1, "l2")[0][:, 0])
+    pa = rng.integers(0, len(rows), 50)
+    pb = rng.integers(0, len(rows), 50)
+    for m in ("l2", "cos", "dot"):
+        assert np.array_equal(bits(db.pair_metrics(pa, pb, m)), bits(O.pairs_fp64(M, pa, pb, m)))
+    db.close()
+
+
+def test_stale_case_reads_current_rows(golden):
+    # After update(1) and delete(0) the reference tree still answers with
+    # stale or out-of-range indices (golden stale_ref_ids = [1, 1, 3]); the
+    # device store always searches the current rows.
+    S = golden["stale_rows"]
+    db = svdb.Database(kd_dimension=2)
+    for i in range(4):
+        db.insert(S[i], b"s%d" % i)
+    db.update(1, np.array([100.0, 100.0]), b"upd")
+    db.delete(0)
+    cur = np.array([[100.0, 100.0], [20.0, 20.0], [30.0, 30.0]])
+    got = [db.kd_nearest(q) for q in golden["stale_q"]]
+    assert got == [1, 0, 2]
+    assert got == O.topk(cur, golden["stale_q"], 1, "l2")[0][:, 0].tolist()
+    db.close()
+
+
+# ---------------------------------------------------------------- pairs
+def test_pair_metrics_fp64_match_oracle(gauss):
+    X, _, db = gauss
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, X.shape[0], 1000)
+    b = rng.integers(0, X.shape[0], 1000)
+    for m in ("l2", "cos", "dot"):
+        assert np.array_equal(bits(db.pair_metrics(a, b, m)), bits(O.pairs_fp64(X, a, b, m)))
+
+
+# ---------------------------------------------------------------- persistence
+def test_load_reference_file_and_roundtrip(golden):
+    src = os.path.join(ROOT, "tests", "golden", "ref_save_small.bin")
+    db = svdb.Database.load(src, 3)
+    R = golden["save_rows"]
+    assert len(db) == R.shape[0]
+    for i in range(R.shape[0]):
+        u, v = db.read(i)
+        assert u == golden["save_uuids"][i] and np.array_equal(v, R[i])
+    ids, sc = db.search(R[:2], 2, "l2", "fp64")
+    assert ids[:, 0].tolist() == [0, 1]
+    with tempfile.TemporaryDirectory() as td:
+        fn = os.path.join(td, "rt.db")
+        db.save(fn)
+        assert open(fn, "rb").read() == open(src, "rb").read()
+    db.close()
+
+
+def test_standalone_kdtree():
+    rng = np.random.default_rng(9)
+    P = rng.standard_normal((500, 3))
+    t = svdb.StandaloneKDTree(3)
+    for i, p in enumerate(P):
+        t.insert(p, 1000 + i)
+    q = rng.standard_normal((20, 3))
+    got = [t.nearest(x) for x in q]
+    exp = (O.topk(P, q, 1, "l2")[0][:, 0] + 1000).tolist()
+    assert got == exp
+    t.insert(q[0], 1000)  # overwrite label 1000 with the query itself
+    assert t.nearest(q[0]) == 1000
+    t.close()
+
+
+# ---------------------------------------------------------------- C2 at full size
+@pytest.mark.parametrize("metric", ["l2", "cos", "dot"])
+def test_c2_full_store_subset_of_queries(metric):
+    # 1,000,000 x 128 N(0,1) (seed 1); 64 of the 10,000 queries (seed 2);
+    # k = 10.  The full query set runs in bench.py.
+    say(f"C2 1Mx128 {metric}: building")
+    X = np.random.default_rng(1).standard_normal((1_000_000, 128))
+    Q = np.random.default_rng(2).standard_normal((10_000, 128))[:64]
+    db = svdb.Database(kd_dimension=128)
+    db.bulk_insert(X)
+    ids, sc = db.search(Q, 10, metric, "fp64")
+    rid, rsc = O.topk(X, Q, 10, metric, threads=16)
+    assert_same(ids, sc, rid, rsc, metric)
+    db.close()
