@@ -118,6 +118,15 @@ enum { SVDB_K_SCAN_FP64 = 0, SVDB_K_SCAN_LOWP = 1, SVDB_K_SHORTLIST = 2, SVDB_K_
 int svdb_set_profiling(VectorDatabase* db, int on); /* also resets the counters */
 int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_COUNT] */);
 
+/* Device memory for callers of svdb_search_device, on the engine's device
+ * and HIP runtime (a second HIP runtime in the process, e.g. one bundled by
+ * a framework, cannot share it).  kind: 0 host→device, 1 device→host,
+ * 2 device→device.  svdb_device_sync waits for all work on the device. */
+void* svdb_device_alloc(size_t bytes);
+void svdb_device_free(void* ptr);
+int svdb_device_copy(void* dst, const void* src, size_t bytes, int kind);
+int svdb_device_sync(void);
+
 const char* svdb_last_error(void);
 
 #ifdef __cplusplus

@@ -400,6 +400,42 @@ float dot_product(Vector vec1, Vector vec2) { return legacy_metric(vec1, vec2, S
 
 const char* svdb_last_error(void) { return svdb::last_error(); }
 
+void* svdb_device_alloc(size_t bytes) {
+  if (svdb::device_init() < 0) return nullptr;
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    svdb::set_error(SVDB_ENOMEM, "hipMalloc failed");
+    return nullptr;
+  }
+  return p;
+}
+
+void svdb_device_free(void* ptr) {
+  if (ptr && svdb::device_init() >= 0) (void)hipFree(ptr);
+}
+
+int svdb_device_copy(void* dst, const void* src, size_t bytes, int kind) {
+  if (svdb::device_init() < 0) return SVDB_EDEVICE;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                        : kind == 1 ? hipMemcpyDeviceToHost
+                        : kind == 2 ? hipMemcpyDeviceToDevice
+                                    : hipMemcpyDefault;
+  if (kind < 0 || kind > 2) {
+    svdb::set_error(SVDB_EINVAL, "copy kind must be 0, 1 or 2");
+    return SVDB_EINVAL;
+  }
+  if (hipMemcpy(dst, src, bytes, k) != hipSuccess) {
+    svdb::set_error(SVDB_EDEVICE, "hipMemcpy failed");
+    return SVDB_EDEVICE;
+  }
+  return SVDB_OK;
+}
+
+int svdb_device_sync(void) {
+  if (svdb::device_init() < 0) return SVDB_EDEVICE;
+  return hipDeviceSynchronize() == hipSuccess ? SVDB_OK : SVDB_EDEVICE;
+}
+
 int svdb_search(VectorDatabase* db, const double* queries, size_t nq, size_t dim, const size_t* k,
                 const int* metric, int precision, size_t out_stride, size_t* ids_out,
                 double* scores_out) {
@@ -499,9 +535,7 @@ int svdb_set_budget_bytes(VectorDatabase* db, size_t bytes) {
 int svdb_enable_shadow(VectorDatabase* db, int precision) {
   Engine* e = eng_of(db);
   if (!e) return SVDB_EINVAL;
-  (void)precision;
-  svdb::set_error(SVDB_EINVAL, "low-precision shadow not available in this build");
-  return SVDB_EINVAL;
+  return e->enable_shadow(precision);
 }
 
 int svdb_get_stats(VectorDatabase* db, svdb_stats* out) {

@@ -115,9 +115,13 @@ Engine::~Engine() {
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
   void* bufs[] = {X_, nx2_, xn_, alive_, rank_, s_q_, s_qn_, s_k_, s_pk_, s_ps_, s_ids_, s_sc_,
-                  s_pair_, s_pout_};
+                  s_pair_, s_pout_, s_qb_, s_qp_, s_ck_, s_cs_, s_sk_, s_ss_, s_beta_, s_fail_,
+                  s_err_, s_fq_, s_fid_, s_fsc_, s_fk_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
+                  sh_.rho_bits /* degen_count and degen live inside this block */};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  if (h_fail_) (void)hipHostFree(h_fail_);
+  if (h_stage_) (void)hipHostFree(h_stage_);
   for (auto& e : ev_pool_) (void)hipEventDestroy(e);
   for (auto& ps : prof_pending_) {
     (void)hipEventDestroy(ps.a);
@@ -168,6 +172,13 @@ int Engine::ensure_dim(size_t d) {
   if (d_ == 0) {
     d_ = d;
     ld_ = round_up(d, 8);
+    // BF16 shadow for the MFMA first pass: rows padded to 32, 64, 128 or
+    // 256 elements (the scan keeps a wavefront's queries in registers).
+    size_t ks = 1;
+    while (32 * ks < d) ks *= 2;
+    sh_.dp = 32 * ks;
+    const char* off = std::getenv("SVDB_NO_SHADOW");
+    lowp_ = lowp_wanted_ && ks <= 8 && !(off && *off && *off != '0');
     return ensure_capacity(init_slots_);
   }
   if (d != d_) {
@@ -184,8 +195,12 @@ int Engine::ensure_capacity(size_t need) {
     set_error(SVDB_EINVAL, "store is limited to 2^32-65 slots");
     return SVDB_EINVAL;
   }
-  size_t nc = round_up(std::max({need, cap_ * 2, init_slots_}), 64);
-  nc = std::min(nc, round_up((size_t)UINT32_MAX - 128, 64) - 64);
+  size_t nc = round_up(std::max({need, cap_ * 2, init_slots_}), 128);
+  nc = std::min(nc, (size_t)UINT32_MAX / 128 * 128 - 128);
+  if (lowp_) {
+    int rs = alloc_shadow(nc);
+    if (rs) return rs;
+  }
   const size_t nw = nc / 64, ow = cap_ / 64;
   double *X = nullptr, *nx2 = nullptr, *xn = nullptr;
   uint64_t* al = nullptr;
@@ -222,6 +237,84 @@ int Engine::ensure_capacity(size_t need) {
   h_alive_.resize(nw, 0);
   h_rank_.resize(nw, 0);
   dirty_word_ = 0;  // the new bitmap arrays hold nothing yet
+  return SVDB_OK;
+}
+
+// (Re)allocates the BF16 shadow for `nc` slots: new slots start "dead"
+// (zero row, aux values that make every ordering key +inf); the first
+// nphys_ slots are copied from the old shadow.
+int Engine::alloc_shadow(size_t nc) {
+  Shadow n = sh_;
+  int rc;
+  const size_t xb = nc * sh_.dp * sizeof(uint16_t), fb = nc * sizeof(float);
+  if ((rc = dev_alloc((void**)&n.Xb, xb))) return rc;
+  if ((rc = dev_alloc((void**)&n.nx, fb)) || (rc = dev_alloc((void**)&n.xn, fb)) ||
+      (rc = dev_alloc((void**)&n.inv, fb)) || (rc = dev_alloc((void**)&n.pen, fb))) {
+    dev_free(n.Xb, xb);
+    if (n.nx != sh_.nx) dev_free(n.nx, fb);
+    if (n.xn != sh_.xn) dev_free(n.xn, fb);
+    if (n.inv != sh_.inv) dev_free(n.inv, fb);
+    return rc;
+  }
+  if (!sh_.rho_bits) {
+    if ((rc = dev_alloc((void**)&n.rho_bits, 3 * sizeof(uint32_t) + kDegenCap * sizeof(uint32_t))))
+      return rc;
+    n.degen_count = n.rho_bits + 1;
+    n.degen = n.rho_bits + 3;
+    HIPCHK(hipMemsetAsync(n.rho_bits, 0, 3 * sizeof(uint32_t), stream_), "memset");
+  }
+  HIPCHK(hipMemsetAsync(n.Xb, 0, xb, stream_), "memset");
+  HIPCHK(launch_shadow_init(n, 0, (uint32_t)nc, stream_), "k_shadow_init");
+  if (sh_.Xb && nphys_) {
+    HIPCHK(hipMemcpyAsync(n.Xb, sh_.Xb, nphys_ * sh_.dp * sizeof(uint16_t), hipMemcpyDeviceToDevice, stream_), "grow");
+    HIPCHK(hipMemcpyAsync(n.nx, sh_.nx, nphys_ * sizeof(float), hipMemcpyDeviceToDevice, stream_), "grow");
+    HIPCHK(hipMemcpyAsync(n.xn, sh_.xn, nphys_ * sizeof(float), hipMemcpyDeviceToDevice, stream_), "grow");
+    HIPCHK(hipMemcpyAsync(n.inv, sh_.inv, nphys_ * sizeof(float), hipMemcpyDeviceToDevice, stream_), "grow");
+    HIPCHK(hipMemcpyAsync(n.pen, sh_.pen, nphys_ * sizeof(float), hipMemcpyDeviceToDevice, stream_), "grow");
+  }
+  HIPCHK(hipStreamSynchronize(stream_), "grow");
+  if (sh_.Xb) {
+    const size_t oxb = cap_ * sh_.dp * sizeof(uint16_t), ofb = cap_ * sizeof(float);
+    dev_free(sh_.Xb, oxb);
+    dev_free(sh_.nx, ofb);
+    dev_free(sh_.xn, ofb);
+    dev_free(sh_.inv, ofb);
+    dev_free(sh_.pen, ofb);
+  }
+  sh_ = n;
+  return SVDB_OK;
+}
+
+// Quantizes slots [first, first+n) (or the device slot list) into the shadow.
+int Engine::shadow_rows(uint32_t first, uint32_t n, const uint32_t* dslots) {
+  if (!lowp_ || n == 0) return SVDB_OK;
+  HIPCHK(launch_quantize_rows(sh_, X_, ld_, nx2_, xn_, dslots, first, n, (uint32_t)d_, stream_),
+         "k_quantize_rows");
+  return SVDB_OK;
+}
+
+int Engine::enable_shadow(int precision) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (precision != SVDB_PREC_BF16 && precision != SVDB_PREC_AUTO) {
+    set_error(SVDB_EINVAL, "only the BF16 shadow is available");
+    return SVDB_EINVAL;
+  }
+  lowp_wanted_ = true;
+  if (lowp_ || d_ == 0) return SVDB_OK;
+  if (sh_.dp > 256) {
+    set_error(SVDB_EINVAL, "BF16 first pass supports dimensions up to 256");
+    return SVDB_EINVAL;
+  }
+  int rc = alloc_shadow(cap_);
+  if (rc) return rc;
+  lowp_ = true;
+  if ((rc = shadow_rows(0, (uint32_t)nphys_, nullptr))) return rc;
+  for (size_t s = 0; s < nphys_; s++)
+    if (!((h_alive_[s >> 6] >> (s & 63)) & 1ull)) {
+      HIPCHK(hipMemsetAsync(sh_.Xb + s * sh_.dp, 0, sh_.dp * sizeof(uint16_t), stream_), "memset");
+      HIPCHK(launch_shadow_init(sh_, (uint32_t)s, 1, stream_), "k_shadow_init");
+    }
+  HIPCHK(hipStreamSynchronize(stream_), "enable_shadow");
   return SVDB_OK;
 }
 
@@ -270,12 +363,30 @@ int Engine::append(const double* rows, size_t n, size_t lds, size_t dim, uint32_
   if ((rc = ensure_capacity(nphys_ + n))) return rc;
   const size_t s0 = nphys_;
   if (n) {
-    if (ld_ != d_)
-      HIPCHK(hipMemsetAsync(X_ + s0 * ld_, 0, n * ld_ * sizeof(double), stream_), "memset");
-    HIPCHK(hipMemcpy2DAsync(X_ + s0 * ld_, ld_ * sizeof(double), rows, lds * sizeof(double),
-                            d_ * sizeof(double), n, hipMemcpyHostToDevice, stream_), "row upload");
+    // Rows go through a pinned staging buffer in large chunks, already in
+    // the device layout (stride ld, zero padding), one DMA per chunk.
+    const size_t row_b = ld_ * sizeof(double);
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(n, (64u << 20) / row_b));
+    if (!h_stage_ || h_stage_n_ < chunk * row_b) {
+      if (h_stage_) (void)hipHostFree(h_stage_);
+      h_stage_ = nullptr;
+      HIPCHK(hipHostMalloc(&h_stage_, chunk * row_b), "hipHostMalloc");
+      h_stage_n_ = chunk * row_b;
+    }
+    for (size_t r0 = 0; r0 < n; r0 += chunk) {
+      const size_t m = std::min(chunk, n - r0);
+      double* st = static_cast<double*>(h_stage_);
+      for (size_t r = 0; r < m; r++) {
+        std::memcpy(st + r * ld_, rows + (r0 + r) * lds, d_ * sizeof(double));
+        if (ld_ != d_) std::memset(st + r * ld_ + d_, 0, (ld_ - d_) * sizeof(double));
+      }
+      HIPCHK(hipMemcpyAsync(X_ + (s0 + r0) * ld_, st, m * row_b, hipMemcpyHostToDevice, stream_),
+             "row upload");
+      HIPCHK(hipStreamSynchronize(stream_), "row upload");
+    }
     HIPCHK(launch_row_norms(X_, ld_, nullptr, (uint32_t)s0, (uint32_t)n, (uint32_t)d_, nx2_, xn_,
                             stream_), "k_row_norms");
+    if ((rc = shadow_rows((uint32_t)s0, (uint32_t)n, nullptr))) return rc;
   }
   for (size_t s = s0; s < s0 + n; s++) {
     if ((s & 63) == 0) h_rank_[s >> 6] = (uint32_t)nalive_;
@@ -305,6 +416,8 @@ int Engine::overwrite(size_t logical, const double* row, size_t dim) {
   HIPCHK(hipMemcpyAsync(X_ + (size_t)s * ld_, row, d_ * sizeof(double), hipMemcpyHostToDevice,
                         stream_), "row upload");
   HIPCHK(launch_row_norms(X_, ld_, nullptr, s, 1, (uint32_t)d_, nx2_, xn_, stream_), "k_row_norms");
+  int rc = shadow_rows(s, 1, nullptr);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(stream_), "overwrite");
   return SVDB_OK;
 }
@@ -312,12 +425,19 @@ int Engine::overwrite(size_t logical, const double* row, size_t dim) {
 int Engine::erase(size_t logical) {
   std::lock_guard<std::mutex> g(mu_);
   if (!ok_) return SVDB_EDEVICE;
+  if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
   if (logical >= nalive_) {
     set_error(SVDB_EINVAL, "index out of range");
     return SVDB_EINVAL;
   }
   const uint32_t s = select_slot(logical);
   const size_t w = s >> 6, nw = (nphys_ + 63) / 64;
+  if (lowp_) {
+    // the first pass must never pick a tombstone
+    HIPCHK(hipMemsetAsync(sh_.Xb + (size_t)s * sh_.dp, 0, sh_.dp * sizeof(uint16_t), stream_), "memset");
+    HIPCHK(launch_shadow_init(sh_, s, 1, stream_), "k_shadow_init");
+    HIPCHK(hipStreamSynchronize(stream_), "erase");
+  }
   h_alive_[w] &= ~(1ull << (s & 63));
   for (size_t v = w + 1; v < nw; v++) h_rank_[v]--;
   nalive_--;
@@ -405,6 +525,8 @@ int Engine::set_shortlist(size_t L) {
 
 // Queries per launch of the scan: bounds the partial-list scratch.
 static constexpr size_t kMaxBatch = 8192;
+// Queries per launch of the low-precision pipeline.
+static constexpr size_t kMaxBatchLowp = 16384;
 
 int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk,
                       int metric, int kmax, uint64_t* dids, double* dscores, uint32_t ostride) {
@@ -438,6 +560,116 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
   HIPCHK(launch_merge(m, metric, kmax, stream_), "k_merge");
   prof_end(SVDB_K_MERGE);
   return SVDB_OK;
+}
+
+// Low-precision first pass + shortlist + FP64 rerank for nq device queries
+// (all columns, one metric and k).  Queries the rerank cannot certify are
+// gathered and answered by the FP64 scan.
+int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k,
+                     uint64_t* dids, double* dscores, uint32_t ostride) {
+  int rc;
+  const size_t dp = sh_.dp;
+  const uint32_t qpb = (uint32_t)lowp_queries_per_block(dp);
+  const uint32_t nq_pad = (uint32_t)round_up(nq, qpb);
+  // Shortlist: the configured width (64 by default) for small k; 256 once
+  // k passes 32, where the k-th and L-th neighbours need more room than the
+  // first-pass error bound leaves in a 2k-wide list.
+  const uint32_t L = (uint32_t)std::min<size_t>(
+      kMaxShort, std::max<size_t>({shortlist_, (size_t)k, k > 32 ? (size_t)kMaxShort : 0}));
+  const uint32_t nrows_pad = (uint32_t)round_up(nphys_, kLowpTile);
+  // Row slices: at least 8 (each query then has >= 32 candidate lists, so
+  // the lists' own bound sits well past the shortlist's), more when there
+  // are few query blocks, each at least 4 tiles long.
+  const uint32_t groups = nq_pad / qpb;
+  const uint32_t tiles = nrows_pad / kLowpTile;
+  uint32_t ns = std::max<uint32_t>(8, (512 + groups - 1) / groups);
+  ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, tiles / 4)));
+  const uint32_t rps = (uint32_t)round_up((tiles + ns - 1) / ns, 1) * kLowpTile;
+  ns = (nrows_pad + rps - 1) / rps;
+  const uint32_t NL = ns * kLowpLanesPerQuery;
+  const size_t ncand = (size_t)nq * NL * kLowpLP;
+  if ((rc = scratch(&s_qb_, &s_qb_n_, (size_t)nq_pad * dp * sizeof(uint16_t)))) return rc;
+  if ((rc = scratch(&s_qp_, &s_qp_n_, (size_t)nq_pad * sizeof(QParams)))) return rc;
+  if ((rc = scratch(&s_ck_, &s_ck_n_, ncand * sizeof(float)))) return rc;
+  if ((rc = scratch(&s_cs_, &s_cs_n_, ncand * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&s_sk_, &s_sk_n_, (size_t)nq * L * sizeof(float)))) return rc;
+  if ((rc = scratch(&s_ss_, &s_ss_n_, (size_t)nq * L * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&s_beta_, &s_beta_n_, (size_t)nq * sizeof(float)))) return rc;
+  if ((rc = scratch(&s_fail_, &s_fail_n_, ((size_t)nq + 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&s_err_, &s_err_n_, (size_t)nq * sizeof(double)))) return rc;
+  if (!h_fail_) HIPCHK(hipHostMalloc((void**)&h_fail_, sizeof(uint32_t)), "hipHostMalloc");
+  if ((rc = sync_alive())) return rc;
+
+  uint32_t* fail_count = (uint32_t*)s_fail_;
+  uint32_t* fail_list = fail_count + 1;
+  HIPCHK(hipMemsetAsync(fail_count, 0, sizeof(uint32_t), stream_), "memset");
+  HIPCHK(hipMemsetAsync(s_err_, 0, (size_t)nq * sizeof(double), stream_), "memset");
+  HIPCHK(launch_query_prep(dQ, ldq, nq, nq_pad, (uint32_t)d_, dp, metric, sh_.rho_bits,
+                           (uint16_t*)s_qb_, (QParams*)s_qp_, stream_), "k_query_prep");
+  const float* w0 = metric == kL2 ? sh_.nx : metric == kDot ? sh_.xn : sh_.inv;
+  const float* w1 = sh_.pen;
+  LowpScanArgs sa{sh_.Xb, dp, w0, w1, (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, rps,
+                  nrows_pad, NL, (float*)s_ck_, (uint32_t*)s_cs_};
+  prof_begin(SVDB_K_SCAN_LOWP);
+  HIPCHK(launch_scan_bf16(sa, metric, ns, stream_), "k_scan_bf16");
+  prof_end(SVDB_K_SCAN_LOWP);
+  ShortlistArgs sl{(const float*)s_ck_, (const uint32_t*)s_cs_, nq, NL, L, (uint32_t*)s_ss_,
+                   (float*)s_sk_, (float*)s_beta_};
+  prof_begin(SVDB_K_SHORTLIST);
+  HIPCHK(launch_shortlist(sl, stream_), "k_shortlist");
+  prof_end(SVDB_K_SHORTLIST);
+  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
+  RerankArgs ra{sv, dQ, ldq, (uint32_t)d_, (const QParams*)s_qp_, (const uint32_t*)s_ss_,
+                (const float*)s_sk_, (const float*)s_beta_, sh_.degen, sh_.degen_count, w0, w1,
+                nq, L, (uint32_t)k, ostride, dids, dscores, fail_count, fail_list,
+                (double*)s_err_};
+  prof_begin(SVDB_K_RERANK);
+  HIPCHK(launch_rerank(ra, metric, stream_), "k_rerank");
+  prof_end(SVDB_K_RERANK);
+  HIPCHK(hipMemcpyAsync(h_fail_, fail_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "copy");
+  HIPCHK(hipStreamSynchronize(stream_), "lowp");
+  const uint32_t nf = *h_fail_;
+  if (nf > 0) {
+    if ((rc = scratch(&s_fq_, &s_fq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
+    if ((rc = scratch(&s_fid_, &s_fid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
+    if ((rc = scratch(&s_fsc_, &s_fsc_n_, (size_t)nf * k * sizeof(double)))) return rc;
+    if ((rc = scratch(&s_fk_, &s_fk_n_, (size_t)nf * sizeof(int)))) return rc;
+    std::vector<int> hk(nf, k);
+    HIPCHK(hipMemcpyAsync(s_fk_, hk.data(), nf * sizeof(int), hipMemcpyHostToDevice, stream_), "copy");
+    HIPCHK(launch_gather_rows(dQ, ldq, fail_list, nf, (uint32_t)d_, (double*)s_fq_, d_, stream_),
+           "k_gather_rows");
+    rc = run_exact((const double*)s_fq_, d_, nf, (uint32_t)d_, (const int*)s_fk_, metric, k,
+                   (uint64_t*)s_fid_, (double*)s_fsc_, (uint32_t)k);
+    if (rc) return rc;
+    HIPCHK(launch_scatter_results((const uint64_t*)s_fid_, (const double*)s_fsc_, (uint32_t)k,
+                                  fail_list, nf, dids, dscores, ostride, stream_),
+           "k_scatter_results");
+    HIPCHK(hipStreamSynchronize(stream_), "fallback");
+  }
+  std::vector<double> err(nq);
+  HIPCHK(hipMemcpy(err.data(), s_err_, nq * sizeof(double), hipMemcpyDeviceToHost), "copy");
+  double mx = 0.0;
+  for (double e : err) mx = std::max(mx, e);
+  st_.last_max_rel_err = mx;
+  st_.certified += nq - nf;
+  st_.fallbacks += nf;
+  return SVDB_OK;
+}
+
+bool Engine::use_lowp(int precision, size_t D, int* rc) {
+  *rc = SVDB_OK;
+  if (precision == SVDB_PREC_FP64) return false;
+  if (precision == SVDB_PREC_FP8) {
+    set_error(SVDB_EINVAL, "FP8 first pass not available for this store");
+    *rc = SVDB_EINVAL;
+    return false;
+  }
+  const bool ok = lowp_ && D == d_;
+  if (precision == SVDB_PREC_BF16 && !ok) {
+    set_error(SVDB_EINVAL, "BF16 first pass needs the shadow (dimension <= 256) and all columns");
+    *rc = SVDB_EINVAL;
+  }
+  return ok;
 }
 
 static int check_search(size_t d, size_t D, int metric, int precision) {
@@ -475,14 +707,19 @@ int Engine::search_device(const double* dQ, size_t nq, size_t ldq, size_t D, siz
     set_error(SVDB_EINVAL, "k must be in [1, 128] and ldq >= compared columns");
     return SVDB_EINVAL;
   }
-  const size_t nb = std::min(nq, kMaxBatch);
+  const size_t nb = std::min(nq, std::max(kMaxBatch, kMaxBatchLowp));
   if ((rc = scratch(&s_k_, &s_k_n_, nb * sizeof(int)))) return rc;
   std::vector<int> hk(nb, (int)k);
   HIPCHK(hipMemcpyAsync(s_k_, hk.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
-  for (size_t q0 = 0; q0 < nq; q0 += kMaxBatch) {
-    const uint32_t n = (uint32_t)std::min(kMaxBatch, nq - q0);
-    rc = run_exact(dQ + q0 * ldq, ldq, n, (uint32_t)D, (const int*)s_k_, metric, (int)k,
-                   dids + q0 * k, dscores + q0 * k, (uint32_t)k);
+  const bool lowp = use_lowp(precision, D, &rc) && nphys_ > 0;
+  if (rc) return rc;
+  const size_t step = lowp ? kMaxBatchLowp : kMaxBatch;
+  for (size_t q0 = 0; q0 < nq; q0 += step) {
+    const uint32_t n = (uint32_t)std::min(step, nq - q0);
+    rc = lowp ? run_lowp(dQ + q0 * ldq, ldq, n, metric, (int)k, dids + q0 * k, dscores + q0 * k,
+                         (uint32_t)k)
+              : run_exact(dQ + q0 * ldq, ldq, n, (uint32_t)D, (const int*)s_k_, metric, (int)k,
+                          dids + q0 * k, dscores + q0 * k, (uint32_t)k);
     if (rc) return rc;
   }
   HIPCHK(hipStreamSynchronize(stream_), "search");
@@ -523,6 +760,9 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   int rc;
   for (size_t i = 0; i < nq; i++)
     if ((rc = check_search(d_, D, metric[i], precision))) return rc;
+  const bool lowp = use_lowp(precision, D, &rc) && nphys_ > 0;
+  if (rc) return rc;
+  const size_t step = lowp ? kMaxBatchLowp : kMaxBatch;
 
   std::vector<double> hq;
   std::vector<int> hk;
@@ -533,8 +773,8 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
     sel.clear();
     for (size_t i = 0; i < nq; i++)
       if (metric[i] == m) sel.push_back(i);
-    for (size_t b0 = 0; b0 < sel.size(); b0 += kMaxBatch) {
-      const size_t n = std::min(kMaxBatch, sel.size() - b0);
+    for (size_t b0 = 0; b0 < sel.size(); b0 += step) {
+      const size_t n = std::min(step, sel.size() - b0);
       size_t kmax = 0;
       hq.assign(n * D, 0.0);
       hk.resize(n);
@@ -550,8 +790,10 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
       if ((rc = scratch(&s_sc_, &s_sc_n_, n * kmax * sizeof(double)))) return rc;
       HIPCHK(hipMemcpyAsync(s_q_, hq.data(), n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
       HIPCHK(hipMemcpyAsync(s_k_, hk.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
-      rc = run_exact((const double*)s_q_, D, (uint32_t)n, (uint32_t)D, (const int*)s_k_, m,
-                     (int)kmax, (uint64_t*)s_ids_, (double*)s_sc_, (uint32_t)kmax);
+      rc = lowp ? run_lowp((const double*)s_q_, D, (uint32_t)n, m, (int)kmax, (uint64_t*)s_ids_,
+                           (double*)s_sc_, (uint32_t)kmax)
+                : run_exact((const double*)s_q_, D, (uint32_t)n, (uint32_t)D, (const int*)s_k_, m,
+                            (int)kmax, (uint64_t*)s_ids_, (double*)s_sc_, (uint32_t)kmax);
       if (rc) return rc;
       hid.resize(n * kmax);
       hsc.resize(n * kmax);

@@ -26,6 +26,7 @@
 
 #include "svdb.h"
 #include "svdb_kernels.h"
+#include "svdb_lowp.h"
 
 namespace svdb {
 
@@ -85,6 +86,8 @@ class Engine {
   int set_budget(size_t bytes);
   int set_shortlist(size_t L);
   int set_profiling(bool on);
+  int enable_shadow(int precision);
+  bool shadow_on() const { return lowp_; }
   void kernel_times(KernelTime* out);
   EngineStats stats();
 
@@ -99,6 +102,11 @@ class Engine {
   int scratch(void** p, size_t* have, size_t need);
   int run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk, int metric,
                 int kmax, uint64_t* dids, double* dscores, uint32_t ostride);
+  int run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k, uint64_t* dids,
+               double* dscores, uint32_t ostride);
+  bool use_lowp(int precision, size_t D, int* rc);
+  int alloc_shadow(size_t cap);
+  int shadow_rows(uint32_t first, uint32_t n, const uint32_t* dslots);
   uint32_t select_slot(size_t logical) const;
   void prof_begin(int which);
   void prof_end(int which);
@@ -122,6 +130,12 @@ class Engine {
   bool has_deletes_ = false;
   size_t dirty_word_ = SIZE_MAX;  // first bitmap word not yet on the device
 
+  // low-precision shadow
+  bool lowp_ = false;
+  bool lowp_wanted_ = true;
+  Shadow sh_;
+  uint32_t* s_slot1_ = nullptr;  // one-slot staging for kill/requantize
+
   size_t dev_bytes_ = 0;
   size_t budget_ = SIZE_MAX;
   size_t shortlist_ = 64;
@@ -136,6 +150,22 @@ class Engine {
   void* s_sc_ = nullptr;     size_t s_sc_n_ = 0;
   void* s_pair_ = nullptr;   size_t s_pair_n_ = 0;
   void* s_pout_ = nullptr;   size_t s_pout_n_ = 0;
+  void* s_qb_ = nullptr;     size_t s_qb_n_ = 0;
+  void* s_qp_ = nullptr;     size_t s_qp_n_ = 0;
+  void* s_ck_ = nullptr;     size_t s_ck_n_ = 0;
+  void* s_cs_ = nullptr;     size_t s_cs_n_ = 0;
+  void* s_sk_ = nullptr;     size_t s_sk_n_ = 0;
+  void* s_ss_ = nullptr;     size_t s_ss_n_ = 0;
+  void* s_beta_ = nullptr;   size_t s_beta_n_ = 0;
+  void* s_fail_ = nullptr;   size_t s_fail_n_ = 0;
+  void* s_err_ = nullptr;    size_t s_err_n_ = 0;
+  void* s_fq_ = nullptr;     size_t s_fq_n_ = 0;
+  void* s_fid_ = nullptr;    size_t s_fid_n_ = 0;
+  void* s_fsc_ = nullptr;    size_t s_fsc_n_ = 0;
+  void* s_fk_ = nullptr;     size_t s_fk_n_ = 0;
+  uint32_t* h_fail_ = nullptr;  // pinned: fail count
+  void* h_stage_ = nullptr;     // pinned: row upload staging
+  size_t h_stage_n_ = 0;
 
   struct ProfSlot {
     int kind;

@@ -1,0 +1,127 @@
+// svdb_lowp.h — low-precision first pass + FP64 rerank (internal).
+//
+// Shadow of the store (per slot):
+//   Xb      bf16 row, zero-padded to dp = round_up(d, 32)
+//   aux_nx  fp32 ‖x‖² rounded down      (Euclidean ordering key)
+//   aux_xn  fp32 ‖x‖ rounded up         (dot ordering key)
+//   aux_inv fp32 1/‖x‖                  (cosine ordering key)
+//   aux_pen fp32 0, or +inf for rows the first pass must never pick
+// Dead slots and "degenerate" rows (zero, tiny, huge or non-finite) carry
+// values that make every ordering key +inf; degenerate rows go to a short
+// list that every query's rerank includes, so they are never missed.
+//
+// First-pass error bound (per query, for every row x):
+//   |s - q·x| <= c_q·‖x‖ + δ,  s = fp32-accumulated q̂·x̂ (bf16 q̂, x̂)
+//   c_q = γ‖q̂‖(1+ρ) + ‖q̂-q‖ + ρ‖q̂‖(1+γ) + FP64-key slack,
+//   ρ = max over stored rows of ‖x̂-x‖/‖x‖, γ = (d+32)·2^-24, δ = d·2^-124.
+// Each metric's fp32 ordering key O is built so that the exact rank key of
+// every row satisfies R >= O·scale_q + off_q; a query is certified when its
+// k-th exact FP64 key is below that bound for the best row left out.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "svdb_kernels.h"
+
+namespace svdb {
+
+constexpr int kLowpWaves = 8;                      // wavefronts per block
+constexpr int kLowpQF = 4;                         // 16-query fragments per wavefront
+constexpr int kLowpQPB = kLowpWaves * kLowpQF * 16;  // queries per block (512)
+constexpr int kLowpTile = 128;                     // rows per LDS tile
+constexpr int kLowpLP = 8;                         // entries per per-lane list
+constexpr int kLowpLanesPerQuery = 4;              // lists per query per slice
+constexpr int kDegenCap = 64;                      // degenerate rows kept in the list
+constexpr int kMaxShort = 256;                     // widest shortlist
+
+struct Shadow {
+  uint16_t* Xb = nullptr;   // [cap][dp]
+  size_t dp = 0;            // padded row length (elements)
+  float* nx = nullptr;
+  float* xn = nullptr;
+  float* inv = nullptr;
+  float* pen = nullptr;
+  uint32_t* rho_bits = nullptr;   // fp32 bits of ρ (running max)
+  uint32_t* degen_count = nullptr;
+  uint32_t* degen = nullptr;      // [kDegenCap]
+};
+
+// Per-query parameters written by the prep kernel.
+struct QParams {
+  float P;        // epilogue multiplier (L2: a_q; dot: -c'_q; cos: unused)
+  int flags;      // 1 = degenerate query: FP64 only
+  double scale;   // R >= O*scale + off
+  double off;
+  double qn;      // FP64 ‖q‖ (canonical chain)
+  double nq2;     // FP64 ‖q‖²
+};
+
+struct LowpScanArgs {
+  const uint16_t* Xb;
+  size_t dp;
+  const float* w0;       // per-row array for the key (nx / xn / inv)
+  const float* w1;       // cosine: pen
+  const uint16_t* Qb;    // [nq_pad][dp]
+  const QParams* qp;
+  uint32_t nq;
+  uint32_t rows_per_slice;  // multiple of kLowpTile
+  uint32_t nrows_pad;       // rows readable in Xb / aux (multiple of kLowpTile)
+  uint32_t NL;              // lists per query = slices * 4
+  float* cand_key;          // [nq][NL][LP]
+  uint32_t* cand_slot;
+};
+
+struct ShortlistArgs {
+  const float* cand_key;
+  const uint32_t* cand_slot;
+  uint32_t nq, NL, L;
+  uint32_t* short_slot;     // [nq][L]  (UINT32_MAX when fewer)
+  float* short_key;         // [nq][L]
+  float* beta;              // [nq] bound on the ordering key of rows left out
+};
+
+struct RerankArgs {
+  StoreView st;
+  const double* Q;          // FP64 queries, stride ldq
+  size_t ldq;
+  uint32_t d;
+  const QParams* qp;
+  const uint32_t* short_slot;
+  const float* short_key;
+  const float* beta;
+  const uint32_t* degen;
+  const uint32_t* degen_count;
+  const float* w0;          // ordering-key row arrays (for the error report)
+  const float* w1;
+  uint32_t nq, L, k;
+  uint32_t ostride;
+  uint64_t* ids;
+  double* scores;
+  uint32_t* fail_count;     // queries not certified
+  uint32_t* fail_list;      // their indices
+  double* max_rel_err;      // [nq] first-pass score error of the returned scores
+};
+
+// Queries per block of the scan for a padded row length dp.
+int lowp_queries_per_block(size_t dp);
+
+hipError_t launch_shadow_init(const Shadow& sh, uint32_t first, uint32_t n, hipStream_t s);
+hipError_t launch_quantize_rows(const Shadow& sh, const double* X, size_t ldx, const double* nx2,
+                                const double* xn, const uint32_t* slots, uint32_t first,
+                                uint32_t n, uint32_t d, hipStream_t s);
+hipError_t launch_kill_rows(const Shadow& sh, const uint32_t* slots, uint32_t n, hipStream_t s);
+hipError_t launch_query_prep(const double* Q, size_t ldq, uint32_t nq, uint32_t nq_pad,
+                             uint32_t d, size_t dp, int metric, const uint32_t* rho_bits,
+                             uint16_t* Qb, QParams* qp, hipStream_t s);
+hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, hipStream_t s);
+hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s);
+hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s);
+hipError_t launch_gather_rows(const double* src, size_t lds, const uint32_t* idx, uint32_t n,
+                              uint32_t d, double* dst, size_t ldd, hipStream_t s);
+hipError_t launch_scatter_results(const uint64_t* ids, const double* sc, uint32_t k,
+                                  const uint32_t* idx, uint32_t n, uint64_t* out_ids,
+                                  double* out_sc, uint32_t ostride, hipStream_t s);
+
+}  // namespace svdb

@@ -1,0 +1,744 @@
+// svdb_lowp.hip — BF16 MFMA first pass, shortlist, FP64 rerank with
+// certification (gfx950).  See svdb_lowp.h for the shadow layout and the
+// error bound.  Built with -ffp-contract=off like the FP64 kernels.
+#include "svdb_lowp.h"
+
+#include <math.h>
+
+namespace svdb {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------- helpers
+
+__device__ __forceinline__ uint16_t bf16_of(double x) {
+  float f = __double2float_rn(x);
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (u & 0x007FFFFFu) ? (uint16_t)0x7FC0 : (uint16_t)(u >> 16);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  uint16_t b = (uint16_t)(u >> 16);
+  if ((b & 0x7F80u) == 0) b &= 0x8000u;  // no bf16 subnormals: flush to zero
+  return b;
+}
+
+__device__ __forceinline__ double bf16_val(uint16_t b) {
+  return (double)__uint_as_float((uint32_t)b << 16);
+}
+
+__device__ __forceinline__ uint64_t rank_bits64(double v) {
+  v = v + 0.0;
+  uint64_t u = (uint64_t)__double_as_longlong(v);
+  if (v != v) u = 0x7FF8000000000000ull;
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double rank_val64(uint64_t u) {
+  uint64_t b = (u >> 63) ? (u & 0x7FFFFFFFFFFFFFFFull) : ~u;
+  return __longlong_as_double((long long)b);
+}
+
+__device__ __forceinline__ uint32_t rank_bits32(float v) {
+  uint32_t u = __float_as_uint(v + 0.0f);
+  if (v != v) u = 0x7FC00000u;
+  return (u >> 31) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ bool lt2(uint64_t ka, uint32_t sa, uint64_t kb, uint32_t sb) {
+  return (ka < kb) | ((ka == kb) & (sa < sb));
+}
+
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Wave-distributed sorted list (entry e in lane e%64, register e/64).
+template <int KPL>
+struct WL {
+  uint64_t k[KPL];
+  uint32_t s[KPL];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      k[p] = ~0ull;
+      s[p] = ~0u;
+    }
+  }
+  __device__ __forceinline__ void at(int e, uint64_t& tk, uint32_t& ts) const {
+    int p = e >> 6, ln = e & 63;
+    uint64_t a = k[0];
+    uint32_t b = s[0];
+#pragma unroll
+    for (int q = 1; q < KPL; q++)
+      if (p == q) {
+        a = k[q];
+        b = s[q];
+      }
+    tk = uni64(__shfl(a, ln));
+    ts = uni32(__shfl(b, ln));
+  }
+  __device__ __forceinline__ void insert(uint64_t K, uint32_t S, int kk, int lane) {
+    int pos = 0;
+#pragma unroll
+    for (int p = 0; p < KPL; p++) pos += __popcll(__ballot(lt2(k[p], s[p], K, S)));
+    if (pos >= kk) return;
+    uint64_t pk[KPL];
+    uint32_t ps[KPL];
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      pk[p] = __shfl_up(k[p], 1);
+      ps[p] = __shfl_up(s[p], 1);
+    }
+#pragma unroll
+    for (int p = 1; p < KPL; p++) {
+      uint64_t ck = __shfl(k[p - 1], 63);
+      uint32_t cs = __shfl(s[p - 1], 63);
+      if (lane == 0) {
+        pk[p] = ck;
+        ps[p] = cs;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      int e = p * 64 + lane;
+      if (e > pos) {
+        k[p] = pk[p];
+        s[p] = ps[p];
+      } else if (e == pos) {
+        k[p] = K;
+        s[p] = S;
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------- shadow rows
+
+__global__ void k_shadow_init(Shadow sh, uint32_t first, uint32_t n) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t slot = first + r;
+  sh.nx[slot] = INFINITY;
+  sh.xn[slot] = -INFINITY;
+  sh.inv[slot] = 0.0f;
+  sh.pen[slot] = INFINITY;
+}
+
+hipError_t launch_shadow_init(const Shadow& sh, uint32_t first, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shadow_init, dim3((n + 255) / 256), dim3(256), 0, s, sh, first, n);
+  return hipGetLastError();
+}
+
+__device__ void kill_shadow_row(const Shadow& sh, uint32_t slot) {
+  uint16_t* xb = sh.Xb + (size_t)slot * sh.dp;
+  for (size_t i = 0; i < sh.dp; i++) xb[i] = 0;
+  sh.nx[slot] = INFINITY;
+  sh.xn[slot] = -INFINITY;
+  sh.inv[slot] = 0.0f;
+  sh.pen[slot] = INFINITY;
+}
+
+// One thread per row (after k_row_norms has written nx2 / xn).
+__global__ void k_quantize_rows(Shadow sh, const double* __restrict__ X, size_t ldx,
+                                const double* __restrict__ nx2, const double* __restrict__ xnorm,
+                                const uint32_t* __restrict__ slots, uint32_t first, uint32_t n,
+                                uint32_t d) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t slot = slots ? slots[r] : first + r;
+  const double* x = X + (size_t)slot * ldx;
+  uint16_t* xb = sh.Xb + (size_t)slot * sh.dp;
+  double esq = 0.0, maxabs = 0.0;
+  bool finite = true;
+  for (uint32_t i = 0; i < d; i++) {
+    double v = x[i];
+    finite = finite && isfinite(v);
+    maxabs = fmax(maxabs, fabs(v));
+    uint16_t b = bf16_of(v);
+    xb[i] = b;
+    double e = bf16_val(b) - v;
+    esq = fma(e, e, esq);
+  }
+  for (size_t i = d; i < sh.dp; i++) xb[i] = 0;
+  const double norm = xnorm[slot];
+  const double lim = 1152921504606846976.0;  // 2^60
+  const bool degenerate = !finite || !(norm >= 1.0 / lim) || !(norm <= lim) || !(maxabs <= lim);
+  if (degenerate) {
+    kill_shadow_row(sh, slot);
+    uint32_t cnt = atomicAdd(sh.degen_count, 0u);
+    bool present = false;
+    for (uint32_t j = 0; j < cnt && j < (uint32_t)kDegenCap; j++) present |= sh.degen[j] == slot;
+    if (!present) {
+      uint32_t idx = atomicAdd(sh.degen_count, 1u);
+      if (idx < (uint32_t)kDegenCap) sh.degen[idx] = slot;
+    }
+    return;
+  }
+  sh.nx[slot] = __double2float_rd(nx2[slot]);
+  sh.xn[slot] = __double2float_ru(norm);
+  sh.inv[slot] = __double2float_rn(1.0 / norm);
+  sh.pen[slot] = 0.0f;
+  const double up = 1.0 + 0x1p-40;
+  const double ratio = (sqrt(esq) * up) / norm * up;
+  atomicMax(sh.rho_bits, __float_as_uint(__double2float_ru(ratio)));
+}
+
+hipError_t launch_quantize_rows(const Shadow& sh, const double* X, size_t ldx, const double* nx2,
+                                const double* xn, const uint32_t* slots, uint32_t first,
+                                uint32_t n, uint32_t d, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_quantize_rows, dim3((n + 127) / 128), dim3(128), 0, s, sh, X, ldx, nx2, xn,
+                     slots, first, n, d);
+  return hipGetLastError();
+}
+
+__global__ void k_kill_rows(Shadow sh, const uint32_t* __restrict__ slots, uint32_t n) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) kill_shadow_row(sh, slots[r]);
+}
+
+hipError_t launch_kill_rows(const Shadow& sh, const uint32_t* slots, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_kill_rows, dim3((n + 63) / 64), dim3(64), 0, s, sh, slots, n);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- query prep
+//
+// One thread per query: FP64 ‖q‖ (the canonical chain), bf16 q̂ scaled by
+// -2 (Euclidean) or -1 (dot, cosine) — exact power-of-two scalings — and
+// the per-query bound constants.
+
+__global__ void k_query_prep(const double* __restrict__ Q, size_t ldq, uint32_t nq,
+                             uint32_t nq_pad, uint32_t d, size_t dp, int metric,
+                             const uint32_t* __restrict__ rho_bits, uint16_t* __restrict__ Qb,
+                             QParams* __restrict__ qp) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq_pad) return;
+  uint16_t* qb = Qb + (size_t)i * dp;
+  QParams p{};
+  if (i >= nq) {
+    for (size_t c = 0; c < dp; c++) qb[c] = 0;
+    p.flags = 1;
+    qp[i] = p;
+    return;
+  }
+  const double* q = Q + (size_t)i * ldq;
+  const double sc = metric == kL2 ? -2.0 : -1.0;
+  double nq2 = 0.0, esq = 0.0, hsq = 0.0, maxabs = 0.0;
+  bool finite = true;
+  for (uint32_t c = 0; c < d; c++) {
+    double v = q[c];
+    nq2 = fma(v, v, nq2);
+    finite = finite && isfinite(v);
+    maxabs = fmax(maxabs, fabs(v));
+    uint16_t b = bf16_of(v);
+    double h = bf16_val(b);
+    double e = h - v;
+    esq = fma(e, e, esq);
+    hsq = fma(h, h, hsq);
+    qb[c] = bf16_of(h * sc);
+  }
+  for (size_t c = d; c < dp; c++) qb[c] = 0;
+  const double qn = sqrt(nq2);
+  const double lim = 1152921504606846976.0;  // 2^60
+  const double up = 1.0 + 0x1p-40;
+  const double rho = (double)__uint_as_float(*rho_bits);
+  const double gam = (double)(d + 32) * 0x1p-24;
+  const double qh = sqrt(hsq) * up, eq = sqrt(esq) * up;
+  const double qh1 = qh * (1.0 + rho) * (1.0 + gam) * up;  // >= |s| / ‖x‖
+  const double delta = (double)d * 0x1p-124;
+  const double c = (gam * qh * (1.0 + rho) + eq + rho * qh * (1.0 + gam) +
+                    (double)(d + 8) * 0x1p-52 * qn) * up;
+  p.qn = qn;
+  p.nq2 = nq2;
+  p.flags = (!finite || !(qn >= 1.0 / lim) || !(qn <= lim) || !(maxabs <= lim)) ? 1 : 0;
+  if (metric == kL2) {
+    const double a = 1.0 - c / qn - 0x1p-24 * (1.0 + qh1 / qn) - 0x1p-30;
+    p.P = __double2float_rd(a);
+    p.scale = 1.0;
+    p.off = nq2 - qn * (c + 0x1p-24 * qh1) - 2.0 * delta;
+    if (!(a > 0.0)) p.flags = 1;
+  } else if (metric == kDot) {
+    const double cp = (c + 0x1p-23 * (qh1 + c) * 1.01) * up;
+    p.P = __double2float_rd(-cp);
+    p.scale = 1.0;
+    p.off = -delta;
+  } else {
+    p.P = 0.0f;
+    p.scale = 1.0 / qn;
+    p.off = -(c + 0x1p-23 * qh1 * 1.01 + delta * lim) / qn - (double)(d + 16) * 0x1p-52;
+  }
+  qp[i] = p;
+}
+
+hipError_t launch_query_prep(const double* Q, size_t ldq, uint32_t nq, uint32_t nq_pad,
+                             uint32_t d, size_t dp, int metric, const uint32_t* rho_bits,
+                             uint16_t* Qb, QParams* qp, hipStream_t s) {
+  if (nq_pad == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_query_prep, dim3((nq_pad + 127) / 128), dim3(128), 0, s, Q, ldq, nq, nq_pad,
+                     d, dp, metric, rho_bits, Qb, qp);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- MFMA scan
+//
+// Block: 8 wavefronts, each holding QF fragments of 16 queries in
+// registers (B operands of v_mfma_f32_16x16x32_bf16), so a block serves
+// 128*QF queries; grid (query blocks, row slices).  The slice is streamed
+// in 128-row tiles through LDS (double-buffered, rows padded by 16 bytes so
+// the 16-row A-operand reads hit distinct banks).  For each 16-row subtile
+// a lane receives 4 scores per fragment: rows 4*(lane/16)+r, query
+// lane%16.  The ordering key is one fp32 FMA of the score and a per-row
+// value; keys below the lane's current 8th-best enter the lane's private
+// sorted list (an insertion into 8 registers).  Each lane's list is one
+// candidate list of the query; its 8th entry bounds every row the lane
+// left out.
+
+// Global → registers half of the tile copy: CPT 16-byte chunks per thread
+// (consecutive threads take consecutive chunks of a row: coalesced), plus
+// one float4 of the aux arrays for threads 0..31 (and 32..63 for cosine).
+template <int CPT, int CPR, bool TWO_AUX>
+__device__ __forceinline__ void tile_load(uint4 (&pre)[CPT], float4& preaux, const LowpScanArgs& a,
+                                          uint32_t row0, int tid) {
+#pragma unroll
+  for (int j = 0; j < CPT; j++) {
+    const int c = tid + 512 * j;
+    pre[j] = *reinterpret_cast<const uint4*>(a.Xb + (size_t)(row0 + c / CPR) * a.dp + (c % CPR) * 8);
+  }
+  if (tid < 32) preaux = *reinterpret_cast<const float4*>(a.w0 + row0 + tid * 4);
+  else if (TWO_AUX && tid < 64) preaux = *reinterpret_cast<const float4*>(a.w1 + row0 + (tid - 32) * 4);
+}
+
+// Registers → LDS half: rows land at a stride of SB bytes.
+template <int CPT, int CPR, int SB, int TILE_B, bool TWO_AUX>
+__device__ __forceinline__ void tile_store(const uint4 (&pre)[CPT], const float4& preaux,
+                                           unsigned char* base, int tid) {
+#pragma unroll
+  for (int j = 0; j < CPT; j++) {
+    const int c = tid + 512 * j;
+    *reinterpret_cast<uint4*>(base + (c / CPR) * SB + (c % CPR) * 16) = pre[j];
+  }
+  float* aux = reinterpret_cast<float*>(base + TILE_B);
+  if (tid < 32) *reinterpret_cast<float4*>(aux + tid * 4) = preaux;
+  else if (TWO_AUX && tid < 64) *reinterpret_cast<float4*>(aux + kLowpTile + (tid - 32) * 4) = preaux;
+}
+
+template <int METRIC, int KS, int QF>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_scan_bf16(LowpScanArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  constexpr int DP = 32 * KS;              // padded row length (bf16)
+  constexpr int RB = DP * 2;               // row bytes in HBM
+  constexpr int SB = RB + 16;              // row stride in LDS
+  constexpr int TILE_B = kLowpTile * SB;   // tile bytes
+  constexpr int BUF_B = TILE_B + kLowpTile * 8;  // + two fp32 aux arrays
+  constexpr int CPR = RB / 16;             // 16-byte chunks per row
+  constexpr int CPT = kLowpTile * CPR / 512;  // chunks per thread per tile (= KS)
+  constexpr int LP = kLowpLP;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ql = lane & 15, rg = lane >> 4;
+  const uint32_t qpb = kLowpWaves * QF * 16;
+  const uint32_t qbase = blockIdx.x * qpb + wave * QF * 16;
+  const uint32_t slice = blockIdx.y;
+  const uint32_t r0 = slice * a.rows_per_slice;
+  const uint32_t r1 = min(r0 + a.rows_per_slice, a.nrows_pad);
+  const uint32_t ntiles = r1 > r0 ? (r1 - r0) / kLowpTile : 0;
+
+  // Queries: B operands, epilogue multipliers, lists.
+  bf16x8 B[QF][KS];
+  float P[QF], tau[QF];
+  float lk[QF][LP];
+  uint32_t ls[QF][LP];
+#pragma unroll
+  for (int f = 0; f < QF; f++) {
+    const uint32_t q = qbase + f * 16 + ql;  // < nq_pad (Qb is padded)
+    const uint16_t* qrow = a.Qb + (size_t)q * a.dp + rg * 8;
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) B[f][ks] = *reinterpret_cast<const bf16x8*>(qrow + ks * 32);
+    P[f] = a.qp[min(q, a.nq - 1)].P;
+    tau[f] = INFINITY;
+#pragma unroll
+    for (int j = 0; j < LP; j++) {
+      lk[f][j] = INFINITY;
+      ls[f][j] = ~0u;
+    }
+  }
+
+  // Tile loader: each thread moves CPT 16-byte chunks plus, for threads
+  // 0..63, one float4 of an aux array.
+  uint4 pre[CPT];
+  float4 preaux = make_float4(0.f, 0.f, 0.f, 0.f);
+#define SVDB_LOAD_TILE(T) \
+  tile_load<CPT, CPR, METRIC == kCos>(pre, preaux, a, r0 + (T) * kLowpTile, tid)
+#define SVDB_STORE_TILE(BUF) \
+  tile_store<CPT, CPR, SB, TILE_B, METRIC == kCos>(pre, preaux, lds + (BUF) * BUF_B, tid)
+
+  if (ntiles > 0) {
+    SVDB_LOAD_TILE(0u);
+    SVDB_STORE_TILE(0);
+  }
+  __syncthreads();
+
+  for (uint32_t t = 0; t < ntiles; t++) {
+    const int cur = t & 1;
+    if (t + 1 < ntiles) SVDB_LOAD_TILE(t + 1);
+    const unsigned char* base = lds + cur * BUF_B;
+    const float* aux = reinterpret_cast<const float*>(base + TILE_B);
+    const uint32_t trow = r0 + t * kLowpTile;
+#pragma unroll 2
+    for (int sub = 0; sub < kLowpTile / 16; sub++) {
+      const unsigned char* ap = base + (sub * 16 + ql) * SB + rg * 16;
+      bf16x8 A[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) A[ks] = *reinterpret_cast<const bf16x8*>(ap + ks * 64);
+      f32x4 acc[QF];
+#pragma unroll
+      for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+        for (int f = 0; f < QF; f++)
+          acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
+      const int rl = sub * 16 + rg * 4;  // first of this lane's 4 rows in the tile
+      const float4 w = *reinterpret_cast<const float4*>(aux + rl);
+      float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (METRIC == kCos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+      const uint32_t srow = trow + rl;
+#pragma unroll
+      for (int f = 0; f < QF; f++) {
+        float k4[4];
+        const float wv[4] = {w.x, w.y, w.z, w.w};
+        const float pv[4] = {pen.x, pen.y, pen.z, pen.w};
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          k4[r] = METRIC == kCos ? fmaf(acc[f][r], wv[r], pv[r]) : fmaf(P[f], wv[r], acc[f][r]);
+        const float mn = fminf(fminf(k4[0], k4[1]), fminf(k4[2], k4[3]));
+        if (mn < tau[f]) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            if (k4[r] < lk[f][LP - 1]) {
+              lk[f][LP - 1] = k4[r];
+              ls[f][LP - 1] = srow + r;
+#pragma unroll
+              for (int j = LP - 1; j > 0; j--) {
+                const bool sw = lk[f][j] < lk[f][j - 1];
+                const float tk = sw ? lk[f][j - 1] : lk[f][j];
+                const uint32_t tsl = sw ? ls[f][j - 1] : ls[f][j];
+                lk[f][j - 1] = sw ? lk[f][j] : lk[f][j - 1];
+                ls[f][j - 1] = sw ? ls[f][j] : ls[f][j - 1];
+                lk[f][j] = tk;
+                ls[f][j] = tsl;
+              }
+            }
+          }
+          tau[f] = lk[f][LP - 1];
+        }
+      }
+    }
+    if (t + 1 < ntiles) SVDB_STORE_TILE(cur ^ 1);
+    __syncthreads();
+  }
+#undef SVDB_LOAD_TILE
+#undef SVDB_STORE_TILE
+
+  const uint32_t list = slice * kLowpLanesPerQuery + rg;
+#pragma unroll
+  for (int f = 0; f < QF; f++) {
+    const uint32_t q = qbase + f * 16 + ql;
+    if (q >= a.nq) continue;
+    const size_t o = ((size_t)q * a.NL + list) * LP;
+#pragma unroll
+    for (int j = 0; j < LP; j++) {
+      a.cand_key[o + j] = lk[f][j];
+      a.cand_slot[o + j] = ls[f][j];
+    }
+  }
+}
+
+template <int METRIC, int KS, int QF>
+static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
+  constexpr int SB = 64 * KS + 16;
+  constexpr size_t shm = 2 * ((size_t)kLowpTile * SB + kLowpTile * 8);
+  auto fn = k_scan_bf16<METRIC, KS, QF>;
+  hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+  if (e != hipSuccess) return e;
+  const uint32_t qpb = kLowpWaves * QF * 16;
+  dim3 grid((a.nq + qpb - 1) / qpb, nslices);
+  hipLaunchKernelGGL(fn, grid, dim3(512), shm, s, a);
+  return hipGetLastError();
+}
+
+template <int METRIC>
+static hipError_t scan_bf16_m(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
+  switch (a.dp / 32) {
+    case 1: return scan_bf16_t<METRIC, 1, 4>(a, nslices, s);
+    case 2: return scan_bf16_t<METRIC, 2, 4>(a, nslices, s);
+    case 4: return scan_bf16_t<METRIC, 4, 4>(a, nslices, s);
+    case 8: return scan_bf16_t<METRIC, 8, 2>(a, nslices, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+int lowp_queries_per_block(size_t dp) { return dp / 32 == 8 ? kLowpWaves * 2 * 16 : kLowpQPB; }
+
+hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, hipStream_t s) {
+  if (a.nq == 0 || nslices == 0) return hipSuccess;
+  switch (metric) {
+    case kL2: return scan_bf16_m<kL2>(a, nslices, s);
+    case kCos: return scan_bf16_m<kCos>(a, nslices, s);
+    case kDot: return scan_bf16_m<kDot>(a, nslices, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------- shortlist
+//
+// One wavefront per query: merge the query's NL candidate lists (8 entries
+// each) into the L best by (ordering key, slot), and the bound
+// beta = min(8th entry of every list, L-th merged key): no row outside the
+// shortlist has a smaller ordering key.
+
+__global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= a.nq) return;
+  const uint32_t n = a.NL * kLowpLP;
+  const size_t base = (size_t)q * n;
+  WL<kMaxShort / 64> L;
+  L.init();
+  const int kk = (int)a.L;
+  uint64_t tk = ~0ull;
+  uint32_t ts = ~0u;
+  float lane_min = INFINITY;
+  for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+    const uint32_t c = c0 + lane;
+    float key = INFINITY;
+    uint32_t slot = ~0u;
+    if (c < n) {
+      key = a.cand_key[base + c];
+      slot = a.cand_slot[base + c];
+      if ((c % kLowpLP) == kLowpLP - 1) lane_min = fminf(lane_min, key);
+    }
+    const uint64_t K = slot == ~0u ? ~0ull : (uint64_t)rank_bits32(key);
+    uint64_t m = __ballot(slot != ~0u && lt2(K, slot, tk, ts));
+    while (m) {
+      int src = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      L.insert(uni64(__shfl(K, src)), uni32(__shfl(slot, src)), kk, lane);
+    }
+    L.at(kk - 1, tk, ts);
+  }
+  // min over lanes of the lists' 8th entries
+  for (int off = 32; off >= 1; off >>= 1) lane_min = fminf(lane_min, __shfl_xor(lane_min, off));
+  float oL = INFINITY;
+  if (ts != ~0u) {
+    const uint32_t u = (uint32_t)tk;
+    oL = __uint_as_float((u >> 31) ? (u & 0x7FFFFFFFu) : ~u);
+  }
+#pragma unroll
+  for (int p = 0; p < kMaxShort / 64; p++) {
+    const int e = p * 64 + lane;
+    if (e < kk) {
+      const uint32_t u = (uint32_t)L.k[p];
+      a.short_slot[(size_t)q * a.L + e] = L.s[p];
+      a.short_key[(size_t)q * a.L + e] =
+          L.s[p] == ~0u ? INFINITY : __uint_as_float((u >> 31) ? (u & 0x7FFFFFFFu) : ~u);
+    }
+  }
+  if (lane == 0) a.beta[q] = fminf(lane_min, oL);
+}
+
+hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s) {
+  if (a.nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shortlist, dim3((a.nq + 3) / 4), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- FP64 rerank
+//
+// One wavefront per query.  Lanes take the candidates 64 at a time (the
+// shortlist, then the degenerate rows) and compute their canonical FP64 rank
+// key with the sequential fma chain; the k best by (key, slot) are
+// selected; the query is certified when its k-th key is below the bound
+// beta*scale + off (minus the FP64 rounding slack of that expression and,
+// for Euclidean, of the keys themselves).  Certified queries get their ids
+// and scores written; the others are listed for the FP64 scan.
+
+template <int METRIC>
+__global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= a.nq) return;
+  const QParams qp = a.qp[q];
+  const double* __restrict__ qv = a.Q + (size_t)q * a.ldq;
+  const uint32_t ndeg = min(*a.degen_count, (uint32_t)kDegenCap);
+  const int kk = (int)a.k;
+
+  WL<2> L;
+  L.init();
+  uint64_t tk = ~0ull;
+  uint32_t ts = ~0u;
+  const uint32_t ncand = a.L + ndeg;
+  for (uint32_t j0 = 0; j0 < ncand; j0 += 64) {
+    const uint32_t j = j0 + lane;
+    uint32_t slot = ~0u;
+    if (j < a.L) slot = a.short_slot[(size_t)q * a.L + j];
+    else if (j < ncand) slot = a.degen[j - a.L];
+    bool valid = slot != ~0u && slot < a.st.nrows &&
+                 ((a.st.alive[slot >> 6] >> (slot & 63)) & 1ull);
+    uint64_t K = ~0ull;
+    if (valid) {
+      const double* __restrict__ x = a.st.X + (size_t)slot * a.st.ldx;
+      double acc = 0.0;
+      for (uint32_t c = 0; c < a.d; c++) {
+        if (METRIC == kL2) {
+          double t = x[c] - qv[c];
+          acc = fma(t, t, acc);
+        } else {
+          acc = fma(qv[c], x[c], acc);
+        }
+      }
+      double v = acc;
+      if (METRIC == kDot) v = -acc;
+      else if (METRIC == kCos) v = -(acc / (qp.qn * a.st.xn[slot]));
+      K = rank_bits64(v);
+    }
+    // a degenerate row that is also shortlisted must not enter twice
+    if (valid && j >= a.L) {
+      for (uint32_t t = 0; t < a.L; t++)
+        if (a.short_slot[(size_t)q * a.L + t] == slot) valid = false;
+    }
+    uint64_t m = __ballot(valid && lt2(K, slot, tk, ts));
+    while (m) {
+      int src = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      L.insert(uni64(__shfl(K, src)), uni32(__shfl(slot, src)), kk, lane);
+    }
+    L.at(kk - 1, tk, ts);
+  }
+
+  // certification
+  // more degenerate rows than the list holds: nothing can be certified
+  bool cert = false;
+  if (!qp.flags && *a.degen_count <= (uint32_t)kDegenCap) {
+    const float bo = a.beta[q];
+    double bR;
+    if (bo == INFINITY) {
+      bR = INFINITY;
+    } else {
+      const double bs = (double)bo * qp.scale;
+      bR = bs + qp.off;
+      bR -= 0x1p-40 * (fabs(bs) + fabs(qp.off));
+      if (METRIC == kL2 && bR > 0.0) bR *= 1.0 - (double)(a.d + 4) * 0x1p-52;
+    }
+    const double tau = ts == ~0u ? INFINITY : rank_val64(tk);
+    cert = tau < bR;
+  }
+  if (!cert) {
+    if (lane == 0) {
+      uint32_t idx = atomicAdd(a.fail_count, 1u);
+      a.fail_list[idx] = q;
+    }
+    return;
+  }
+  double err = 0.0;
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    const int e = p * 64 + lane;
+    if (e < kk) {
+      uint64_t id = ~0ull;
+      double sc = __longlong_as_double(0x7FF8000000000000ll);
+      const uint32_t slot = L.s[p];
+      if (slot != ~0u) {
+        const double v = rank_val64(L.k[p]);
+        if (v == v) sc = METRIC == kL2 ? sqrt(v) : (v == 0.0 ? 0.0 : -v);
+        id = a.st.has_deletes
+                 ? (uint64_t)a.st.rank_base[slot >> 6] +
+                       (uint64_t)__popcll(a.st.alive[slot >> 6] & ((1ull << (slot & 63)) - 1ull))
+                 : slot;
+        // first-pass estimate of this score, for the error report
+        float O = INFINITY;
+        for (uint32_t t = 0; t < a.L; t++)
+          if (a.short_slot[(size_t)q * a.L + t] == slot) O = a.short_key[(size_t)q * a.L + t];
+        if (O != INFINITY && sc == sc && sc != 0.0) {
+          const double xn = a.st.xn[slot];
+          double est;
+          if (METRIC == kL2) {
+            const double sp = (double)O - (double)qp.P * (double)a.w0[slot];
+            est = sqrt(fmax(qp.nq2 + xn * xn + sp, 0.0));
+          } else if (METRIC == kDot) {
+            est = -((double)O - (double)qp.P * (double)a.w0[slot]);
+          } else {
+            est = -(double)O / qp.qn;
+          }
+          err = fmax(err, fabs(est - sc) / fabs(sc));
+        }
+      }
+      a.ids[(size_t)q * a.ostride + e] = id;
+      a.scores[(size_t)q * a.ostride + e] = sc;
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) err = fmax(err, __shfl_xor(err, off));
+  if (lane == 0) a.max_rel_err[q] = err;
+}
+
+hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s) {
+  if (a.nq == 0) return hipSuccess;
+  dim3 grid((a.nq + 3) / 4), block(256);
+  switch (metric) {
+    case kL2: hipLaunchKernelGGL(k_rerank<kL2>, grid, block, 0, s, a); break;
+    case kCos: hipLaunchKernelGGL(k_rerank<kCos>, grid, block, 0, s, a); break;
+    case kDot: hipLaunchKernelGGL(k_rerank<kDot>, grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- fallback plumbing
+
+__global__ void k_gather_rows(const double* __restrict__ src, size_t lds,
+                              const uint32_t* __restrict__ idx, uint32_t n, uint32_t d,
+                              double* __restrict__ dst, size_t ldd) {
+  const uint32_t r = blockIdx.x;
+  if (r >= n) return;
+  const double* s = src + (size_t)idx[r] * lds;
+  for (uint32_t c = threadIdx.x; c < d; c += blockDim.x) dst[(size_t)r * ldd + c] = s[c];
+}
+
+hipError_t launch_gather_rows(const double* src, size_t lds, const uint32_t* idx, uint32_t n,
+                              uint32_t d, double* dst, size_t ldd, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_rows, dim3(n), dim3(64), 0, s, src, lds, idx, n, d, dst, ldd);
+  return hipGetLastError();
+}
+
+__global__ void k_scatter_results(const uint64_t* __restrict__ ids, const double* __restrict__ sc,
+                                  uint32_t k, const uint32_t* __restrict__ idx, uint32_t n,
+                                  uint64_t* __restrict__ out_ids, double* __restrict__ out_sc,
+                                  uint32_t ostride) {
+  const uint32_t r = blockIdx.x;
+  if (r >= n) return;
+  const size_t o = (size_t)idx[r] * ostride;
+  for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
+    out_ids[o + j] = ids[(size_t)r * k + j];
+    out_sc[o + j] = sc[(size_t)r * k + j];
+  }
+}
+
+hipError_t launch_scatter_results(const uint64_t* ids, const double* sc, uint32_t k,
+                                  const uint32_t* idx, uint32_t n, uint64_t* out_ids,
+                                  double* out_sc, uint32_t ostride, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_results, dim3(n), dim3(64), 0, s, ids, sc, k, idx, n, out_ids,
+                     out_sc, ostride);
+  return hipGetLastError();
+}
+
+}  // namespace svdb

@@ -30,7 +30,8 @@ EXPORTS = [
     "vector_db_load", "cosine_similarity", "euclidean_distance", "dot_product",
     "svdb_search", "svdb_search_device", "svdb_pair_metrics", "svdb_bulk_insert",
     "svdb_set_shortlist", "svdb_set_budget_bytes", "svdb_enable_shadow", "svdb_get_stats",
-    "svdb_set_profiling", "svdb_get_kernel_times", "svdb_last_error",
+    "svdb_set_profiling", "svdb_get_kernel_times", "svdb_last_error", "svdb_device_alloc",
+    "svdb_device_free", "svdb_device_copy", "svdb_device_sync",
 ]
 
 
@@ -116,6 +117,12 @@ def lib():
     L.svdb_get_kernel_times.restype = C.c_int
     L.svdb_get_kernel_times.argtypes = [DBP, C.POINTER(KernelTime)]
     L.svdb_last_error.restype = C.c_char_p
+    L.svdb_device_alloc.restype = C.c_void_p
+    L.svdb_device_alloc.argtypes = [C.c_size_t]
+    L.svdb_device_free.argtypes = [C.c_void_p]
+    L.svdb_device_copy.restype = C.c_int
+    L.svdb_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    L.svdb_device_sync.restype = C.c_int
     _libc = C.CDLL(None)
     _libc.malloc.restype = C.c_void_p
     _libc.malloc.argtypes = [C.c_size_t]
@@ -172,6 +179,45 @@ def borrowed_vector(row, uuid=b""):
     return v
 
 
+class DeviceArray:
+    """A device buffer from svdb_device_alloc (same HIP runtime as the engine)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = lib().svdb_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise SvdbError(-2, "svdb_device_alloc")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        _chk(lib().svdb_device_copy(d.ptr, a.ctypes.data, a.nbytes, 0), "svdb_device_copy")
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        _chk(lib().svdb_device_copy(out.ctypes.data, self.ptr, self.nbytes, 1), "svdb_device_copy")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().svdb_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync():
+    _chk(lib().svdb_device_sync(), "svdb_device_sync")
+
+
 def legacy_metric(a, b, metric):
     """cosine_similarity / euclidean_distance / dot_product (float results)."""
     f = {L2: "euclidean_distance", COS: "cosine_similarity", DOT: "dot_product"}[_metric(metric)]
@@ -201,7 +247,10 @@ class Database:
             self.p = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __len__(self):
         return self.p.contents.size
@@ -323,4 +372,7 @@ class StandaloneKDTree:
             self.p = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -45,8 +45,9 @@ def test_struct_layouts_match_reference():
 def test_no_gpu_fails_loudly():
     # In a container without a GPU the engine must refuse to start rather than
     # fall back to host code.
-    import torch
-    if torch.cuda.is_available():
+    p = svdb.lib().svdb_device_alloc(8)
+    if p:
+        svdb.lib().svdb_device_free(p)
         pytest.skip("GPU present")
     assert not svdb.lib().vector_db_init(0, 3)
     assert not svdb.lib().kdtree_create(3)

@@ -1,0 +1,157 @@
+"""GPU parity of the BF16 MFMA first pass + FP64 rerank, through the C ABI.
+
+Bar: returned ids and FP64 scores bit-identical to the CPU FP64 oracle (the
+certification guarantees it; uncertified queries are rescanned in FP64).
+Because a broken first pass would still give correct answers through that
+fallback, the tests also require most queries to be certified and the
+first-pass score error to be at BF16 level (not zero, not large).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import svdb
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    a = np.array(a, dtype=np.float64)
+    a[np.isnan(a)] = np.nan
+    return a.view(np.uint64)
+
+
+def assert_same(ids, sc, rid, rsc, what=""):
+    bad = np.argwhere(ids != rid)
+    assert bad.size == 0, f"{what}: ids differ at {bad[:5].tolist()}"
+    assert np.array_equal(bits(sc), bits(rsc)), f"{what}: scores differ"
+
+
+def cluster_mixture(rng, n, d, centers, sigma=0.05, lognorm=0.5):
+    """Rows of the cluster-mixture workload: x = r·u/‖u‖, u = c_j + σz."""
+    j = rng.integers(0, centers.shape[0], n)
+    u = centers[j] + sigma * rng.standard_normal((n, d))
+    return rng.lognormal(0.0, lognorm, (n, 1)) * u / np.linalg.norm(u, axis=1, keepdims=True)
+
+
+@pytest.fixture(scope="module")
+def mix():
+    rng = np.random.default_rng(42)
+    d = 128
+    C = rng.standard_normal((32, d))
+    C /= np.linalg.norm(C, axis=1, keepdims=True)
+    X = cluster_mixture(rng, 300_000, d, C)
+    Q = cluster_mixture(rng, 700, d, C)
+    say("mixture store: 300000x128, 32 clusters (~9400 rows each)")
+    db = svdb.Database(kd_dimension=d)
+    db.bulk_insert(X)
+    yield X, Q, db
+    db.close()
+
+
+@pytest.mark.parametrize("metric", ["l2", "cos", "dot"])
+@pytest.mark.parametrize("k", [1, 10, 100])
+def test_bf16_rerank_matches_oracle(mix, metric, k):
+    X, Q, db = mix
+    say(f"bf16 {metric} k={k}: 700 queries")
+    s0 = db.stats()
+    ids, sc = db.search(Q, k, metric, "bf16")
+    s1 = db.stats()
+    rid, rsc = O.topk(X, Q, k, metric)
+    assert_same(ids, sc, rid, rsc, f"{metric} k={k}")
+    cert = s1["certified"] - s0["certified"]
+    fb = s1["fallbacks"] - s0["fallbacks"]
+    say(f"  certified {cert} fallback {fb} max first-pass rel err {s1['last_max_rel_err']:.2e}")
+    assert cert + fb == Q.shape[0]
+    assert cert >= (0.9 if k < 100 else 0.5) * Q.shape[0]
+    assert 1e-7 < s1["last_max_rel_err"] < 5e-2
+
+
+def test_bf16_gaussian(gauss_lowp=None):
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((250_000, 128))
+    Q = rng.standard_normal((600, 128))
+    db = svdb.Database(kd_dimension=128)
+    db.bulk_insert(X)
+    for m in ("l2", "cos", "dot"):
+        say(f"bf16 gaussian {m}")
+        ids, sc = db.search(Q, 10, m, "bf16")
+        rid, rsc = O.topk(X, Q, 10, m)
+        assert_same(ids, sc, rid, rsc, m)
+    st = db.stats()
+    assert st["certified"] >= 0.9 * st["queries"]
+    db.close()
+
+
+def test_auto_mixed_batch(mix):
+    X, Q, db = mix
+    rng = np.random.default_rng(3)
+    ks = rng.choice([1, 10, 100], size=Q.shape[0])
+    ms = rng.choice(["l2", "cos", "dot"], size=Q.shape[0])
+    say("auto precision, mixed k/metric")
+    ids, sc = db.search(Q, ks, list(ms), "auto")
+    for i in range(0, Q.shape[0], 23):
+        rid, rsc = O.topk(X, Q[i:i + 1], int(ks[i]), str(ms[i]))
+        assert_same(ids[i:i + 1, :ks[i]], sc[i:i + 1, :ks[i]], rid, rsc, f"query {i}")
+
+
+def test_bf16_after_mutations_and_degenerate_rows():
+    rng = np.random.default_rng(12)
+    d = 96  # padded to 128 in the shadow
+    rows = [r for r in rng.standard_normal((20_000, d))]
+    db = svdb.Database(kd_dimension=d)
+    db.bulk_insert(np.array(rows))
+    say("bf16 after 400 mutations incl. zero / huge / NaN rows")
+    specials = [np.zeros(d), np.full(d, 1e30), np.full(d, np.nan), np.full(d, 1e-30)]
+    for step in range(400):
+        op = rng.integers(3)
+        r = specials[step % 4] if step % 50 == 0 else rng.standard_normal(d)
+        if op == 0:
+            db.insert(r)
+            rows.append(r)
+        elif op == 1:
+            i = int(rng.integers(len(rows)))
+            db.update(i, r)
+            rows[i] = r
+        else:
+            i = int(rng.integers(len(rows)))
+            db.delete(i)
+            del rows[i]
+    M = np.array(rows)
+    Q = rng.standard_normal((300, d))
+    Q[0] = 0.0  # degenerate query: answered by the FP64 scan
+    for m in ("l2", "cos", "dot"):
+        ids, sc = db.search(Q, 10, m, "bf16")
+        rid, rsc = O.topk(M, Q, 10, m)
+        assert_same(ids, sc, rid, rsc, m)
+    db.close()
+
+
+def test_kd_nearest_full_dimension_uses_first_pass(golden, c1_data):
+    # kd dimension == vector size: find-nearest goes through the BF16 first
+    # pass and must still equal the reference's own answers.
+    X, Q, _ = c1_data
+    db = svdb.Database(kd_dimension=128)
+    db.bulk_insert(X)
+    got = np.array([db.kd_nearest(q) for q in Q], dtype=np.uint64)
+    assert np.array_equal(got, golden["c1_kd128_ids"])
+    assert db.stats()["certified"] > 200
+    db.close()
+
+
+def test_device_entry_point(mix):
+    X, Q, db = mix
+    dq = svdb.DeviceArray.from_host(Q)
+    dids = svdb.DeviceArray((Q.shape[0], 10), np.uint64)
+    dsc = svdb.DeviceArray((Q.shape[0], 10), np.float64)
+    db.search_device(dq.ptr, Q.shape[0], 128, 10, "dot", "bf16", dids.ptr, dsc.ptr)
+    svdb.device_sync()
+    rid, rsc = O.topk(X, Q, 10, "dot")
+    assert_same(dids.to_host(), dsc.to_host(), rid, rsc, "device")
