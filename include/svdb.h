@@ -121,7 +121,9 @@ int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_C
 /* Device memory for callers of svdb_search_device, on the engine's device
  * and HIP runtime (a second HIP runtime in the process, e.g. one bundled by
  * a framework, cannot share it).  kind: 0 host→device, 1 device→host,
- * 2 device→device.  svdb_device_sync waits for all work on the device. */
+ * 2 device→device (a plain 16-byte-per-lane copy kernel, which reads and
+ * writes `bytes` exactly once).  svdb_device_sync waits for all work on the
+ * device. */
 void* svdb_device_alloc(size_t bytes);
 void svdb_device_free(void* ptr);
 int svdb_device_copy(void* dst, const void* src, size_t bytes, int kind);

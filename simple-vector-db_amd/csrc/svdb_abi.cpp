@@ -424,6 +424,14 @@ int svdb_device_copy(void* dst, const void* src, size_t bytes, int kind) {
     svdb::set_error(SVDB_EINVAL, "copy kind must be 0, 1 or 2");
     return SVDB_EINVAL;
   }
+  if (kind == 2) {  // device → device with a plain copy kernel
+    if (svdb::launch_copy(src, dst, bytes, nullptr) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+      svdb::set_error(SVDB_EDEVICE, "device copy failed");
+      return SVDB_EDEVICE;
+    }
+    return SVDB_OK;
+  }
   if (hipMemcpy(dst, src, bytes, k) != hipSuccess) {
     svdb::set_error(SVDB_EDEVICE, "hipMemcpy failed");
     return SVDB_EDEVICE;

@@ -586,6 +586,8 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, tiles / 4)));
   const uint32_t rps = (uint32_t)round_up((tiles + ns - 1) / ns, 1) * kLowpTile;
   ns = (nrows_pad + rps - 1) / rps;
+  // one set of slices per XCD: a multiple of 8 (trailing slices may be empty)
+  ns = (uint32_t)round_up(ns, 8);
   const uint32_t NL = ns * kLowpLanesPerQuery;
   const size_t ncand = (size_t)nq * NL * kLowpLP;
   if ((rc = scratch(&s_qb_, &s_qb_n_, (size_t)nq_pad * dp * sizeof(uint16_t)))) return rc;

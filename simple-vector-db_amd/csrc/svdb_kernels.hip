@@ -17,6 +17,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace svdb {
 
 // ----------------------------------------------------------------- helpers
@@ -163,6 +165,36 @@ hipError_t launch_query_norms(const double* Q, size_t ldq, uint32_t nq, uint32_t
                               hipStream_t s) {
   if (nq == 0) return hipSuccess;
   hipLaunchKernelGGL(k_query_norms, dim3((nq + 255) / 256), dim3(256), 0, s, Q, ldq, nq, D, qn);
+  return hipGetLastError();
+}
+
+// Plain device copy, 16 bytes per lane, grid-stride (also the known-traffic
+// reference for hardware byte counters: it reads and writes `bytes` once).
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_copy1(const unsigned char* __restrict__ src, unsigned char* __restrict__ dst,
+                        size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
+  const size_t n16 = aligned ? bytes / 16 : 0;
+  if (n16) {
+    const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4*)src,
+                       (uint4*)dst, n16);
+  }
+  const size_t rest = bytes - n16 * 16;
+  if (rest)
+    hipLaunchKernelGGL(k_copy1, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, s,
+                       (const unsigned char*)src + n16 * 16, (unsigned char*)dst + n16 * 16, rest);
   return hipGetLastError();
 }
 

@@ -345,8 +345,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int lane = tid & 63, wave = tid >> 6;
   const int ql = lane & 15, rg = lane >> 4;
   const uint32_t qpb = kLowpWaves * QF * 16;
-  const uint32_t qbase = blockIdx.x * qpb + wave * QF * 16;
-  const uint32_t slice = blockIdx.y;
+  // XCD-aware block order: the hardware deals consecutive block ids to the
+  // 8 XCDs in turn, so the blocks with id ≡ x (mod 8) all run on XCD x.  Give
+  // each XCD whole row slices (slice ≡ x mod 8) and let all query blocks of
+  // a slice follow one another there: they stream the same rows through
+  // that XCD's L2 instead of every XCD reading every row.
+  const uint32_t nqb = (a.nq + qpb - 1) / qpb;
+  const uint32_t xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+  const uint32_t qbase = (local % nqb) * qpb + wave * QF * 16;
+  const uint32_t slice = (local / nqb) * 8 + xcd;
   const uint32_t r0 = slice * a.rows_per_slice;
   const uint32_t r1 = min(r0 + a.rows_per_slice, a.nrows_pad);
   const uint32_t ntiles = r1 > r0 ? (r1 - r0) / kLowpTile : 0;
@@ -470,7 +477,9 @@ static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, hipStream
   hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
   if (e != hipSuccess) return e;
   const uint32_t qpb = kLowpWaves * QF * 16;
-  dim3 grid((a.nq + qpb - 1) / qpb, nslices);
+  // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid.
+  if (nslices % 8) return hipErrorInvalidValue;
+  dim3 grid(((a.nq + qpb - 1) / qpb) * nslices);
   hipLaunchKernelGGL(fn, grid, dim3(512), shm, s, a);
   return hipGetLastError();
 }
