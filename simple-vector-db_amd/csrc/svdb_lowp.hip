@@ -286,227 +286,6 @@ hipError_t launch_query_prep(const double* Q, size_t ldq, uint32_t nq, uint32_t 
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------- MFMA scan
-//
-// Block: 8 wavefronts, each holding QF fragments of 16 queries in
-// registers (B operands of v_mfma_f32_16x16x32_bf16), so a block serves
-// 128*QF queries; grid (query blocks, row slices).  The slice is streamed
-// in 128-row tiles through LDS (double-buffered, rows padded by 16 bytes so
-// the 16-row A-operand reads hit distinct banks).  For each 16-row subtile
-// a lane receives 4 scores per fragment: rows 4*(lane/16)+r, query
-// lane%16.  The ordering key is one fp32 FMA of the score and a per-row
-// value; keys below the lane's current 8th-best enter the lane's private
-// sorted list (an insertion into 8 registers).  Each lane's list is one
-// candidate list of the query; its 8th entry bounds every row the lane
-// left out.
-
-// Global → registers half of the tile copy: CPT 16-byte chunks per thread
-// (consecutive threads take consecutive chunks of a row: coalesced), plus
-// one float4 of the aux arrays for threads 0..31 (and 32..63 for cosine).
-template <int CPT, int CPR, bool TWO_AUX>
-__device__ __forceinline__ void tile_load(uint4 (&pre)[CPT], float4& preaux, const LowpScanArgs& a,
-                                          uint32_t row0, int tid) {
-#pragma unroll
-  for (int j = 0; j < CPT; j++) {
-    const int c = tid + 512 * j;
-    pre[j] = *reinterpret_cast<const uint4*>(a.Xb + (size_t)(row0 + c / CPR) * a.dp + (c % CPR) * 8);
-  }
-  if (tid < 32) preaux = *reinterpret_cast<const float4*>(a.w0 + row0 + tid * 4);
-  else if (TWO_AUX && tid < 64) preaux = *reinterpret_cast<const float4*>(a.w1 + row0 + (tid - 32) * 4);
-}
-
-// Registers → LDS half: rows land at a stride of SB bytes.
-template <int CPT, int CPR, int SB, int TILE_B, bool TWO_AUX>
-__device__ __forceinline__ void tile_store(const uint4 (&pre)[CPT], const float4& preaux,
-                                           unsigned char* base, int tid) {
-#pragma unroll
-  for (int j = 0; j < CPT; j++) {
-    const int c = tid + 512 * j;
-    *reinterpret_cast<uint4*>(base + (c / CPR) * SB + (c % CPR) * 16) = pre[j];
-  }
-  float* aux = reinterpret_cast<float*>(base + TILE_B);
-  if (tid < 32) *reinterpret_cast<float4*>(aux + tid * 4) = preaux;
-  else if (TWO_AUX && tid < 64) *reinterpret_cast<float4*>(aux + kLowpTile + (tid - 32) * 4) = preaux;
-}
-
-template <int METRIC, int KS, int QF>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_scan_bf16(LowpScanArgs a) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-  constexpr int DP = 32 * KS;              // padded row length (bf16)
-  constexpr int RB = DP * 2;               // row bytes in HBM
-  constexpr int SB = RB + 16;              // row stride in LDS
-  constexpr int TILE_B = kLowpTile * SB;   // tile bytes
-  constexpr int BUF_B = TILE_B + kLowpTile * 8;  // + two fp32 aux arrays
-  constexpr int CPR = RB / 16;             // 16-byte chunks per row
-  constexpr int CPT = kLowpTile * CPR / 512;  // chunks per thread per tile (= KS)
-  constexpr int LP = kLowpLP;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int ql = lane & 15, rg = lane >> 4;
-  const uint32_t qpb = kLowpWaves * QF * 16;
-  // XCD-aware block order: the hardware deals consecutive block ids to the
-  // 8 XCDs in turn, so the blocks with id ≡ x (mod 8) all run on XCD x.  Give
-  // each XCD whole row slices (slice ≡ x mod 8) and let all query blocks of
-  // a slice follow one another there: they stream the same rows through
-  // that XCD's L2 instead of every XCD reading every row.
-  const uint32_t nqb = (a.nq + qpb - 1) / qpb;
-  const uint32_t xcd = blockIdx.x % 8, local = blockIdx.x / 8;
-  const uint32_t qbase = (local % nqb) * qpb + wave * QF * 16;
-  const uint32_t slice = (local / nqb) * 8 + xcd;
-  const uint32_t r0 = slice * a.rows_per_slice;
-  const uint32_t r1 = min(r0 + a.rows_per_slice, a.nrows_pad);
-  const uint32_t ntiles = r1 > r0 ? (r1 - r0) / kLowpTile : 0;
-
-  // Queries: B operands, epilogue multipliers, lists.
-  bf16x8 B[QF][KS];
-  float P[QF], tau[QF];
-  float lk[QF][LP];
-  uint32_t ls[QF][LP];
-#pragma unroll
-  for (int f = 0; f < QF; f++) {
-    const uint32_t q = qbase + f * 16 + ql;  // < nq_pad (Qb is padded)
-    const uint16_t* qrow = a.Qb + (size_t)q * a.dp + rg * 8;
-#pragma unroll
-    for (int ks = 0; ks < KS; ks++) B[f][ks] = *reinterpret_cast<const bf16x8*>(qrow + ks * 32);
-    P[f] = a.qp[min(q, a.nq - 1)].P;
-    tau[f] = INFINITY;
-#pragma unroll
-    for (int j = 0; j < LP; j++) {
-      lk[f][j] = INFINITY;
-      ls[f][j] = ~0u;
-    }
-  }
-
-  // Tile loader: each thread moves CPT 16-byte chunks plus, for threads
-  // 0..63, one float4 of an aux array.
-  uint4 pre[CPT];
-  float4 preaux = make_float4(0.f, 0.f, 0.f, 0.f);
-#define SVDB_LOAD_TILE(T) \
-  tile_load<CPT, CPR, METRIC == kCos>(pre, preaux, a, r0 + (T) * kLowpTile, tid)
-#define SVDB_STORE_TILE(BUF) \
-  tile_store<CPT, CPR, SB, TILE_B, METRIC == kCos>(pre, preaux, lds + (BUF) * BUF_B, tid)
-
-  if (ntiles > 0) {
-    SVDB_LOAD_TILE(0u);
-    SVDB_STORE_TILE(0);
-  }
-  __syncthreads();
-
-  for (uint32_t t = 0; t < ntiles; t++) {
-    const int cur = t & 1;
-    if (t + 1 < ntiles) SVDB_LOAD_TILE(t + 1);
-    const unsigned char* base = lds + cur * BUF_B;
-    const float* aux = reinterpret_cast<const float*>(base + TILE_B);
-    const uint32_t trow = r0 + t * kLowpTile;
-#pragma unroll 2
-    for (int sub = 0; sub < kLowpTile / 16; sub++) {
-      const unsigned char* ap = base + (sub * 16 + ql) * SB + rg * 16;
-      bf16x8 A[KS];
-#pragma unroll
-      for (int ks = 0; ks < KS; ks++) A[ks] = *reinterpret_cast<const bf16x8*>(ap + ks * 64);
-      f32x4 acc[QF];
-#pragma unroll
-      for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < KS; ks++)
-#pragma unroll
-        for (int f = 0; f < QF; f++)
-          acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
-      const int rl = sub * 16 + rg * 4;  // first of this lane's 4 rows in the tile
-      const float4 w = *reinterpret_cast<const float4*>(aux + rl);
-      float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (METRIC == kCos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
-      const uint32_t srow = trow + rl;
-#pragma unroll
-      for (int f = 0; f < QF; f++) {
-        float k4[4];
-        const float wv[4] = {w.x, w.y, w.z, w.w};
-        const float pv[4] = {pen.x, pen.y, pen.z, pen.w};
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          k4[r] = METRIC == kCos ? fmaf(acc[f][r], wv[r], pv[r]) : fmaf(P[f], wv[r], acc[f][r]);
-        const float mn = fminf(fminf(k4[0], k4[1]), fminf(k4[2], k4[3]));
-        if (mn < tau[f]) {
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            if (k4[r] < lk[f][LP - 1]) {
-              lk[f][LP - 1] = k4[r];
-              ls[f][LP - 1] = srow + r;
-#pragma unroll
-              for (int j = LP - 1; j > 0; j--) {
-                const bool sw = lk[f][j] < lk[f][j - 1];
-                const float tk = sw ? lk[f][j - 1] : lk[f][j];
-                const uint32_t tsl = sw ? ls[f][j - 1] : ls[f][j];
-                lk[f][j - 1] = sw ? lk[f][j] : lk[f][j - 1];
-                ls[f][j - 1] = sw ? ls[f][j] : ls[f][j - 1];
-                lk[f][j] = tk;
-                ls[f][j] = tsl;
-              }
-            }
-          }
-          tau[f] = lk[f][LP - 1];
-        }
-      }
-    }
-    if (t + 1 < ntiles) SVDB_STORE_TILE(cur ^ 1);
-    __syncthreads();
-  }
-#undef SVDB_LOAD_TILE
-#undef SVDB_STORE_TILE
-
-  const uint32_t list = slice * kLowpLanesPerQuery + rg;
-#pragma unroll
-  for (int f = 0; f < QF; f++) {
-    const uint32_t q = qbase + f * 16 + ql;
-    if (q >= a.nq) continue;
-    const size_t o = ((size_t)q * a.NL + list) * LP;
-#pragma unroll
-    for (int j = 0; j < LP; j++) {
-      a.cand_key[o + j] = lk[f][j];
-      a.cand_slot[o + j] = ls[f][j];
-    }
-  }
-}
-
-template <int METRIC, int KS, int QF>
-static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
-  constexpr int SB = 64 * KS + 16;
-  constexpr size_t shm = 2 * ((size_t)kLowpTile * SB + kLowpTile * 8);
-  auto fn = k_scan_bf16<METRIC, KS, QF>;
-  hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-  if (e != hipSuccess) return e;
-  const uint32_t qpb = kLowpWaves * QF * 16;
-  // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid.
-  if (nslices % 8) return hipErrorInvalidValue;
-  dim3 grid(((a.nq + qpb - 1) / qpb) * nslices);
-  hipLaunchKernelGGL(fn, grid, dim3(512), shm, s, a);
-  return hipGetLastError();
-}
-
-template <int METRIC>
-static hipError_t scan_bf16_m(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
-  switch (a.dp / 32) {
-    case 1: return scan_bf16_t<METRIC, 1, 4>(a, nslices, s);
-    case 2: return scan_bf16_t<METRIC, 2, 4>(a, nslices, s);
-    case 4: return scan_bf16_t<METRIC, 4, 4>(a, nslices, s);
-    case 8: return scan_bf16_t<METRIC, 8, 2>(a, nslices, s);
-  }
-  return hipErrorInvalidValue;
-}
-
-int lowp_queries_per_block(size_t dp) { return dp / 32 == 8 ? kLowpWaves * 2 * 16 : kLowpQPB; }
-
-hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, hipStream_t s) {
-  if (a.nq == 0 || nslices == 0) return hipSuccess;
-  switch (metric) {
-    case kL2: return scan_bf16_m<kL2>(a, nslices, s);
-    case kCos: return scan_bf16_m<kCos>(a, nslices, s);
-    case kDot: return scan_bf16_m<kDot>(a, nslices, s);
-  }
-  return hipErrorInvalidValue;
-}
-
 // ------------------------------------------------------------- shortlist
 //
 // One wavefront per query: merge the query's NL candidate lists (8 entries

@@ -1,0 +1,242 @@
+// svdb_scan_bf16.hip — the BF16 MFMA first pass over the store's shadow
+// (gfx950).  Built with -ffp-contract=off; every fma below is explicit.
+//
+// Block: 8 wavefronts, each holding QF fragments of 16 queries in registers
+// as B operands of v_mfma_f32_16x16x32_bf16, so a block serves 128*QF
+// queries.  Grid: query blocks × row slices, ordered so each XCD owns whole
+// slices (the blocks of one slice share that XCD's L2).  A slice is
+// streamed in 128-row tiles straight into LDS (global_load_lds_dwordx4, no
+// registers), double-buffered, 16-byte chunks XOR-swizzled by row so the
+// 16-row A-operand reads hit distinct banks.  Per 16-row subtile a lane
+// gets 4 scores per fragment — rows 4*(lane/16)+r, query lane%16 — and its
+// fp32 ordering key is one FMA of the score and a per-row value.  Keys below
+// the lane's current 8th best enter the lane's private sorted list (an
+// insertion into 8 registers); each list is one candidate list of its
+// query and its 8th entry bounds every row the lane left out.  Subtiles are
+// software-pipelined: subtile s+1's MFMAs are issued before subtile s's
+// epilogue, whose per-row values were read into registers beforehand.
+#include "svdb_lowp.h"
+
+#include <math.h>
+
+namespace svdb {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
+
+// One 16-byte chunk per lane from global memory into LDS at
+// lds_base + 16*lane (the LDS pointer must be wave-uniform).
+__device__ __forceinline__ void load_lds16(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds(g, (lds_void*)lds_base, 16, 0, 0);
+}
+__device__ __forceinline__ void load_lds4(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds(g, (lds_void*)lds_base, 4, 0, 0);
+}
+
+template <int KS>
+struct TileGeom {
+  static constexpr int CPR = 4 * KS;                 // 16-byte chunks per row
+  static constexpr int TILE_B = kLowpTile * CPR * 16;
+  static constexpr int AUX_B = kLowpTile * 4 * 2;    // two fp32 arrays
+  static constexpr int BUF_B = TILE_B + AUX_B;
+  static constexpr int CPW = kLowpTile * CPR / (64 * kLowpWaves);  // chunk-instrs per wave
+  // LDS chunk of (row, column chunk c): swizzled within the row.
+  __device__ static __forceinline__ int chunk(int row, int c) {
+    return row * CPR + (c ^ (row & (CPR - 1)));
+  }
+};
+
+// Issue the direct-to-LDS loads of one tile (rows row0..row0+127) into buf.
+template <int KS, bool TWO_AUX>
+__device__ __forceinline__ void issue_tile(const LowpScanArgs& a, unsigned char* buf,
+                                           uint32_t row0, int wave, int lane) {
+  using G = TileGeom<KS>;
+#pragma unroll
+  for (int j = 0; j < G::CPW; j++) {
+    const int p0 = (wave + kLowpWaves * j) * 64;  // first LDS chunk of this instruction
+    const int p = p0 + lane;
+    const int row = p / G::CPR, pos = p % G::CPR;
+    const int c = pos ^ (row & (G::CPR - 1));
+    load_lds16(a.Xb + (size_t)(row0 + row) * a.dp + c * 8, buf + p0 * 16);
+  }
+  float* aux = reinterpret_cast<float*>(buf + G::TILE_B);
+  if (wave < 2) load_lds4(a.w0 + row0 + wave * 64 + lane, aux + wave * 64);
+  else if (TWO_AUX && wave < 4)
+    load_lds4(a.w1 + row0 + (wave - 2) * 64 + lane, aux + kLowpTile + (wave - 2) * 64);
+}
+
+template <int METRIC, int KS, int QF>
+__global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  using G = TileGeom<KS>;
+  constexpr int LP = kLowpLP;
+  constexpr bool COS = METRIC == kCos;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ql = lane & 15, rg = lane >> 4;
+  const uint32_t qpb = kLowpWaves * QF * 16;
+  // XCD-aware block order: consecutive block ids go to the 8 XCDs in turn,
+  // so ids ≡ x (mod 8) run on XCD x.  Each XCD gets whole slices
+  // (slice ≡ x mod 8) and all query blocks of a slice follow one another
+  // there, streaming the same rows through that XCD's L2.
+  const uint32_t nqb = (a.nq + qpb - 1) / qpb;
+  const uint32_t xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+  const uint32_t qbase = (local % nqb) * qpb + wave * QF * 16;
+  const uint32_t slice = (local / nqb) * 8 + xcd;
+  const uint32_t r0 = slice * a.rows_per_slice;
+  const uint32_t r1 = min(r0 + a.rows_per_slice, a.nrows_pad);
+  const uint32_t ntiles = r1 > r0 ? (r1 - r0) / kLowpTile : 0;
+
+  bf16x8 B[QF][KS];
+  float P[QF], tau[QF];
+  float lk[QF][LP];
+  uint32_t ls[QF][LP];
+#pragma unroll
+  for (int f = 0; f < QF; f++) {
+    const uint32_t q = qbase + f * 16 + ql;  // < nq_pad (Qb is padded)
+    const uint16_t* qrow = a.Qb + (size_t)q * a.dp + rg * 8;
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) B[f][ks] = *reinterpret_cast<const bf16x8*>(qrow + ks * 32);
+    P[f] = a.qp[min(q, a.nq - 1)].P;
+    tau[f] = INFINITY;
+#pragma unroll
+    for (int j = 0; j < LP; j++) {
+      lk[f][j] = INFINITY;
+      ls[f][j] = ~0u;
+    }
+  }
+
+  // Pending subtile (MFMAs issued, epilogue not yet run).
+  f32x4 pacc[QF];
+  float4 pw = make_float4(0.f, 0.f, 0.f, 0.f), ppen = pw;
+  uint32_t prow = 0;
+  bool pending = false;
+
+  auto epilogue = [&](const f32x4 (&acc)[QF], const float4& w, const float4& pen, uint32_t srow) {
+    const float wv[4] = {w.x, w.y, w.z, w.w};
+    const float pv[4] = {pen.x, pen.y, pen.z, pen.w};
+#pragma unroll
+    for (int f = 0; f < QF; f++) {
+      float k4[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        k4[r] = COS ? fmaf(acc[f][r], wv[r], pv[r]) : fmaf(P[f], wv[r], acc[f][r]);
+      const float mn = fminf(fminf(k4[0], k4[1]), fminf(k4[2], k4[3]));
+      if (mn < tau[f]) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          if (k4[r] < lk[f][LP - 1]) {
+            lk[f][LP - 1] = k4[r];
+            ls[f][LP - 1] = srow + r;
+#pragma unroll
+            for (int j = LP - 1; j > 0; j--) {
+              const bool sw = lk[f][j] < lk[f][j - 1];
+              const float tk = sw ? lk[f][j - 1] : lk[f][j];
+              const uint32_t tsl = sw ? ls[f][j - 1] : ls[f][j];
+              lk[f][j - 1] = sw ? lk[f][j] : lk[f][j - 1];
+              ls[f][j - 1] = sw ? ls[f][j] : ls[f][j - 1];
+              lk[f][j] = tk;
+              ls[f][j] = tsl;
+            }
+          }
+        }
+        tau[f] = lk[f][LP - 1];
+      }
+    }
+  };
+
+  if (ntiles > 0) issue_tile<KS, COS>(a, lds, r0, wave, lane);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  for (uint32_t t = 0; t < ntiles; t++) {
+    unsigned char* cur = lds + (t & 1) * G::BUF_B;
+    if (t + 1 < ntiles) issue_tile<KS, COS>(a, lds + ((t + 1) & 1) * G::BUF_B, r0 + (t + 1) * kLowpTile, wave, lane);
+    const float* aux = reinterpret_cast<const float*>(cur + G::TILE_B);
+    const uint32_t trow = r0 + t * kLowpTile;
+#pragma unroll
+    for (int sub = 0; sub < kLowpTile / 16; sub++) {
+      const int arow = sub * 16 + ql;
+      bf16x8 A[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++)
+        A[ks] = *reinterpret_cast<const bf16x8*>(cur + G::chunk(arow, ks * 4 + rg) * 16);
+      const int rl = sub * 16 + rg * 4;
+      const float4 w = *reinterpret_cast<const float4*>(aux + rl);
+      float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (COS) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+      f32x4 acc[QF];
+#pragma unroll
+      for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+        for (int f = 0; f < QF; f++)
+          acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
+      if (pending) epilogue(pacc, pw, ppen, prow);
+#pragma unroll
+      for (int f = 0; f < QF; f++) pacc[f] = acc[f];
+      pw = w;
+      ppen = pen;
+      prow = trow + rl;
+      pending = true;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  if (pending) epilogue(pacc, pw, ppen, prow);
+
+  const uint32_t list = slice * kLowpLanesPerQuery + rg;
+#pragma unroll
+  for (int f = 0; f < QF; f++) {
+    const uint32_t q = qbase + f * 16 + ql;
+    if (q >= a.nq) continue;
+    const size_t o = ((size_t)q * a.NL + list) * LP;
+#pragma unroll
+    for (int j = 0; j < LP; j++) {
+      a.cand_key[o + j] = lk[f][j];
+      a.cand_slot[o + j] = ls[f][j];
+    }
+  }
+}
+
+template <int METRIC, int KS, int QF>
+static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
+  constexpr size_t shm = 2 * (size_t)TileGeom<KS>::BUF_B;
+  auto fn = k_scan_bf16<METRIC, KS, QF>;
+  hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+  if (e != hipSuccess) return e;
+  const uint32_t qpb = kLowpWaves * QF * 16;
+  // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid.
+  if (nslices % 8) return hipErrorInvalidValue;
+  dim3 grid(((a.nq + qpb - 1) / qpb) * nslices);
+  hipLaunchKernelGGL(fn, grid, dim3(512), shm, s, a);
+  return hipGetLastError();
+}
+
+template <int METRIC>
+static hipError_t scan_bf16_m(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
+  switch (a.dp / 32) {
+    case 1: return scan_bf16_t<METRIC, 1, 4>(a, nslices, s);
+    case 2: return scan_bf16_t<METRIC, 2, 4>(a, nslices, s);
+    case 4: return scan_bf16_t<METRIC, 4, 4>(a, nslices, s);
+    case 8: return scan_bf16_t<METRIC, 8, 2>(a, nslices, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+int lowp_queries_per_block(size_t dp) { return dp / 32 == 8 ? kLowpWaves * 2 * 16 : kLowpQPB; }
+
+hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, hipStream_t s) {
+  if (a.nq == 0 || nslices == 0) return hipSuccess;
+  switch (metric) {
+    case kL2: return scan_bf16_m<kL2>(a, nslices, s);
+    case kCos: return scan_bf16_m<kCos>(a, nslices, s);
+    case kDot: return scan_bf16_m<kDot>(a, nslices, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace svdb
