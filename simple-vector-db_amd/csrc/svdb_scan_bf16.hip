@@ -68,8 +68,9 @@ __device__ __forceinline__ void issue_tile(const LowpScanArgs& a, unsigned char*
 
 template <int METRIC, int KS, int QF>
 __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   using G = TileGeom<KS>;
+  __shared__ __attribute__((aligned(16))) unsigned char buf0[G::BUF_B];
+  __shared__ __attribute__((aligned(16))) unsigned char buf1[G::BUF_B];
   constexpr int LP = kLowpLP;
   constexpr bool COS = METRIC == kCos;
 
@@ -147,44 +148,52 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
     }
   };
 
-  if (ntiles > 0) issue_tile<KS, COS>(a, lds, r0, wave, lane);
+  // Two LDS buffers as distinct objects, picked at compile time (tiles are
+  // taken in pairs and the pair loop is unrolled): reads of one buffer then
+  // need not wait for the direct-to-LDS loads still filling the other.
+  if (ntiles > 0) issue_tile<KS, COS>(a, buf0, r0, wave, lane);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-
-  for (uint32_t t = 0; t < ntiles; t++) {
-    unsigned char* cur = lds + (t & 1) * G::BUF_B;
-    if (t + 1 < ntiles) issue_tile<KS, COS>(a, lds + ((t + 1) & 1) * G::BUF_B, r0 + (t + 1) * kLowpTile, wave, lane);
-    const float* aux = reinterpret_cast<const float*>(cur + G::TILE_B);
-    const uint32_t trow = r0 + t * kLowpTile;
+  for (uint32_t t0 = 0; t0 < ntiles; t0 += 2) {
 #pragma unroll
-    for (int sub = 0; sub < kLowpTile / 16; sub++) {
-      const int arow = sub * 16 + ql;
-      bf16x8 A[KS];
+    for (int h = 0; h < 2; h++) {
+      const uint32_t t = t0 + h;
+      if (t >= ntiles) break;
+      const unsigned char* cur = h ? buf1 : buf0;
+      if (t + 1 < ntiles)
+        issue_tile<KS, COS>(a, h ? buf0 : buf1, r0 + (t + 1) * kLowpTile, wave, lane);
+      const uint32_t trow = r0 + t * kLowpTile;
+      const float* aux = reinterpret_cast<const float*>(cur + G::TILE_B);
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++)
-        A[ks] = *reinterpret_cast<const bf16x8*>(cur + G::chunk(arow, ks * 4 + rg) * 16);
-      const int rl = sub * 16 + rg * 4;
-      const float4 w = *reinterpret_cast<const float4*>(aux + rl);
-      float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (COS) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
-      f32x4 acc[QF];
+      for (int sub = 0; sub < kLowpTile / 16; sub++) {
+        const int arow = sub * 16 + ql;
+        bf16x8 A[KS];
 #pragma unroll
-      for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < KS; ks++)
+          A[ks] = *reinterpret_cast<const bf16x8*>(cur + G::chunk(arow, ks * 4 + rg) * 16);
+        const int rl = sub * 16 + rg * 4;
+        const float4 w = *reinterpret_cast<const float4*>(aux + rl);
+        float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (COS) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+        f32x4 acc[QF];
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++)
+        for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int f = 0; f < QF; f++)
-          acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
-      if (pending) epilogue(pacc, pw, ppen, prow);
+        for (int ks = 0; ks < KS; ks++)
 #pragma unroll
-      for (int f = 0; f < QF; f++) pacc[f] = acc[f];
-      pw = w;
-      ppen = pen;
-      prow = trow + rl;
-      pending = true;
+          for (int f = 0; f < QF; f++)
+            acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
+        if (pending) epilogue(pacc, pw, ppen, prow);
+#pragma unroll
+        for (int f = 0; f < QF; f++) pacc[f] = acc[f];
+        pw = w;
+        ppen = pen;
+        prow = trow + rl;
+        pending = true;
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
   }
   if (pending) epilogue(pacc, pw, ppen, prow);
 
@@ -204,10 +213,8 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
 
 template <int METRIC, int KS, int QF>
 static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
-  constexpr size_t shm = 2 * (size_t)TileGeom<KS>::BUF_B;
   auto fn = k_scan_bf16<METRIC, KS, QF>;
-  hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-  if (e != hipSuccess) return e;
+  constexpr size_t shm = 0;  // both tile buffers are static LDS
   const uint32_t qpb = kLowpWaves * QF * 16;
   // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid.
   if (nslices % 8) return hipErrorInvalidValue;
