@@ -118,6 +118,44 @@ enum { SVDB_K_SCAN_FP64 = 0, SVDB_K_SCAN_LOWP = 1, SVDB_K_SHORTLIST = 2, SVDB_K_
 int svdb_set_profiling(VectorDatabase* db, int on); /* also resets the counters */
 int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_COUNT] */);
 
+/* ---- serving loop -------------------------------------------------------
+ * A continuous-batching server over one store.  Searches and mutations are
+ * submitted from any thread and answered in submission order by one device
+ * thread: each tick takes every queued search (any sizes, k and metrics, up
+ * to max_batch queries and while the tick's estimated device memory stays
+ * within budget_bytes) into one batch; mutations are applied between ticks
+ * in order (a search sees exactly the mutations submitted before it).
+ * Requests arriving during a tick join the next one, which is staged on
+ * the host while the current tick runs.  Results equal those of svdb_search
+ * on the same store state.  Stop the server only after every submitted
+ * request has been waited for. */
+typedef struct svdb_server svdb_server;
+typedef struct svdb_server_config {
+    size_t max_batch;    /* queries per tick (default 8192) */
+    size_t budget_bytes; /* device memory for one tick's requests (default 64 GiB) */
+    int precision;       /* SVDB_PREC_* (default AUTO) */
+} svdb_server_config;
+typedef struct svdb_request_info {
+    int64_t submit_us, start_us, done_us; /* CLOCK_MONOTONIC microseconds */
+    uint64_t tick;                        /* tick that served the request */
+    size_t tick_queries;                  /* queries in that tick */
+} svdb_request_info;
+void svdb_server_default_config(svdb_server_config* cfg);
+svdb_server* svdb_server_start(VectorDatabase* db, const svdb_server_config* cfg);
+void svdb_server_stop(svdb_server* s);
+/* Each returns a ticket (>= 0) or a negative SVDB_E* code.  A search larger
+ * than max_batch queries or than the memory budget is refused (SVDB_ENOMEM). */
+int64_t svdb_server_submit_search(svdb_server* s, const double* queries, size_t nq, size_t dim,
+                                  size_t k, int metric);
+int64_t svdb_server_submit_insert(svdb_server* s, const double* row, size_t dim);
+int64_t svdb_server_submit_update(svdb_server* s, size_t index, const double* row, size_t dim);
+int64_t svdb_server_submit_delete(svdb_server* s, size_t index);
+/* Blocks until the request is done.  Search: ids/scores get nq*k entries
+ * (row stride k).  Insert: ids[0] gets the new index.  Returns the request's
+ * status. */
+int svdb_server_wait(svdb_server* s, int64_t ticket, size_t* ids, double* scores,
+                     svdb_request_info* info);
+
 /* Device memory for callers of svdb_search_device, on the engine's device
  * and HIP runtime (a second HIP runtime in the process, e.g. one bundled by
  * a framework, cannot share it).  kind: 0 host→device, 1 device→host,

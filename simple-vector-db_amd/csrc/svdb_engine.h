@@ -134,7 +134,6 @@ class Engine {
   bool lowp_ = false;
   bool lowp_wanted_ = true;
   Shadow sh_;
-  uint32_t* s_slot1_ = nullptr;  // one-slot staging for kill/requantize
 
   size_t dev_bytes_ = 0;
   size_t budget_ = SIZE_MAX;

@@ -89,6 +89,7 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   const uint32_t r0 = slice * a.rows_per_slice;
   const uint32_t r1 = min(r0 + a.rows_per_slice, a.nrows_pad);
   const uint32_t ntiles = r1 > r0 ? (r1 - r0) / kLowpTile : 0;
+  const bool active = qbase < a.nq;  // wave-uniform
 
   bf16x8 B[QF][KS];
   float P[QF], tau[QF];
@@ -164,8 +165,10 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
         issue_tile<KS, COS>(a, h ? buf0 : buf1, r0 + (t + 1) * kLowpTile, wave, lane);
       const uint32_t trow = r0 + t * kLowpTile;
       const float* aux = reinterpret_cast<const float*>(cur + G::TILE_B);
+      // A wave whose queries all lie past nq (a small batch) only helps
+      // fill the tiles.
 #pragma unroll
-      for (int sub = 0; sub < kLowpTile / 16; sub++) {
+      for (int sub = 0; sub < (active ? kLowpTile / 16 : 0); sub++) {
         const int arow = sub * 16 + ql;
         bf16x8 A[KS];
 #pragma unroll

@@ -1,0 +1,409 @@
+// svdb_server.cpp — continuous-batching serving loop over one store.
+//
+// Requests (searches and mutations) enter one FIFO.  A device-owner thread
+// runs ticks: it applies mutations in arrival order, and gathers every
+// search queued behind them (up to max_batch queries, and while the
+// estimated device memory of the tick stays within the budget) into one
+// batch — any mix of sizes, k and metrics — answered by one call into the
+// engine (one first-pass scan per metric present).  Requests that arrive
+// while a tick is on the GPU wait for the next tick; the queries of the
+// next tick are staged (copied and grouped on the host) by a second thread
+// while the current tick runs.  A mutation is a barrier: searches queued
+// before it see the store without it, searches after it see it.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "svdb_engine.h"
+
+namespace svdb {
+
+static int64_t now_us() {
+  return std::chrono::duration_cast<std::chrono::microseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Request {
+  int64_t ticket = 0;
+  int kind = 0;  // 0 search, 1 insert, 2 update, 3 delete
+  std::vector<double> data;
+  size_t nq = 0, k = 0, index = 0, dim = 0;
+  int metric = 0;
+  size_t est_bytes = 0;
+  std::vector<size_t> ids;
+  std::vector<double> scores;
+  int rc = 0;
+  bool done = false;
+  int64_t t_submit = 0, t_start = 0, t_done = 0;
+  uint64_t tick = 0;
+  size_t tick_queries = 0;
+};
+
+// A staged tick: concatenated queries with per-query k and metric.
+struct Tick {
+  std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<double> q;
+  std::vector<size_t> k;
+  std::vector<int> metric;
+  size_t kmax = 0, nq = 0;
+  // A tick is either a run of consecutive mutations (applied in order) or
+  // a batch of searches.
+  std::vector<std::shared_ptr<Request>> mutations;
+};
+
+class Server {
+ public:
+  static constexpr size_t kMaxMutationsPerTick = 1024;
+
+  Server(VectorDatabase* db, const svdb_server_config& cfg) : db_(db), cfg_(cfg) {
+    eng_ = static_cast<Engine*>(db->svdb_engine);
+    stager_ = std::thread([this] { stage_loop(); });
+    exec_ = std::thread([this] { exec_loop(); });
+  }
+
+  ~Server() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_q_.notify_all();
+    cv_tick_.notify_all();
+    stager_.join();
+    exec_.join();
+    // requests still queued are failed, not dropped
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& r : queue_) {
+      r->rc = SVDB_EINVAL;
+      r->done = true;
+    }
+    cv_done_.notify_all();
+  }
+
+  int64_t submit(std::shared_ptr<Request> r) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (stop_) return SVDB_EINVAL;
+    r->ticket = next_++;
+    r->t_submit = now_us();
+    all_[r->ticket] = r;
+    queue_.push_back(r);
+    cv_q_.notify_all();
+    return r->ticket;
+  }
+
+  int wait(int64_t ticket, size_t* ids, double* scores, svdb_request_info* info) {
+    std::shared_ptr<Request> r;
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      auto it = all_.find(ticket);
+      if (it == all_.end()) {
+        set_error(SVDB_EINVAL, "unknown ticket");
+        return SVDB_EINVAL;
+      }
+      r = it->second;
+      cv_done_.wait(g, [&] { return r->done; });
+      all_.erase(ticket);
+    }
+    if (r->kind == 0 && r->rc == SVDB_OK) {
+      if (ids) std::memcpy(ids, r->ids.data(), r->ids.size() * sizeof(size_t));
+      if (scores) std::memcpy(scores, r->scores.data(), r->scores.size() * sizeof(double));
+    }
+    if (r->kind == 1 && ids) ids[0] = r->index;
+    if (info) {
+      info->submit_us = r->t_submit;
+      info->start_us = r->t_start;
+      info->done_us = r->t_done;
+      info->tick = r->tick;
+      info->tick_queries = r->tick_queries;
+    }
+    return r->rc;
+  }
+
+  size_t query_bytes(size_t nq, size_t k) const {
+    // FP64 query + BF16 query + parameters + results + first-pass candidate
+    // lists (at most 512 slices × 4 lists × 8 entries of 8 bytes)
+    const size_t d = eng_->dim() ? eng_->dim() : 1;
+    return nq * (d * 8 + d * 2 + 64 + k * 16 + 512 * 4 * 8 * 8);
+  }
+
+  const svdb_server_config& cfg() const { return cfg_; }
+  size_t dim() const { return eng_->dim(); }
+
+ private:
+  // Stager: forms the next tick while the executor runs the current one.
+  void stage_loop() {
+    for (;;) {
+      std::unique_ptr<Tick> t(new Tick);
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_q_.wait(g, [&] { return stop_ || (!queue_.empty() && !staged_); });
+        if (stop_) return;
+        if (queue_.front()->kind != 0) {
+          while (!queue_.empty() && queue_.front()->kind != 0 &&
+                 t->mutations.size() < kMaxMutationsPerTick) {
+            t->mutations.push_back(queue_.front());
+            queue_.pop_front();
+          }
+        } else {
+          size_t bytes = 0;
+          while (!queue_.empty() && queue_.front()->kind == 0) {
+            auto r = queue_.front();
+            if (r->dim != eng_->dim()) {  // store dimension changed under it
+              queue_.pop_front();
+              r->rc = SVDB_EINVAL;
+              r->done = true;
+              cv_done_.notify_all();
+              continue;
+            }
+            if (!t->reqs.empty() &&
+                (t->nq + r->nq > cfg_.max_batch || bytes + r->est_bytes > cfg_.budget_bytes))
+              break;
+            bytes += r->est_bytes;
+            t->nq += r->nq;
+            t->reqs.push_back(r);
+            queue_.pop_front();
+          }
+        }
+      }
+      if (t->mutations.empty()) {
+        const size_t d = eng_->dim();
+        t->q.resize(t->nq * d);
+        t->k.resize(t->nq);
+        t->metric.resize(t->nq);
+        size_t o = 0;
+        for (auto& r : t->reqs) {
+          std::memcpy(&t->q[o * d], r->data.data(), r->nq * d * sizeof(double));
+          for (size_t i = 0; i < r->nq; i++) {
+            t->k[o + i] = r->k;
+            t->metric[o + i] = r->metric;
+          }
+          t->kmax = std::max(t->kmax, r->k);
+          o += r->nq;
+        }
+      }
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        staged_ = std::move(t);
+      }
+      cv_tick_.notify_all();
+    }
+  }
+
+  void exec_loop() {
+    for (;;) {
+      std::unique_ptr<Tick> t;
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_tick_.wait(g, [&] { return stop_ || staged_; });
+        if (!staged_ && stop_) return;
+        t = std::move(staged_);
+      }
+      cv_q_.notify_all();  // the stager may form the next tick now
+      const int64_t t0 = now_us();
+      const uint64_t tick = ++ticks_;
+      if (!t->mutations.empty()) {
+        for (auto& m : t->mutations) run_mutation(*m);
+        finish(t->mutations, t0, tick, 0);
+        continue;
+      }
+      const size_t d = eng_->dim();
+      std::vector<size_t> ids(t->nq * t->kmax);
+      std::vector<double> sc(t->nq * t->kmax);
+      int rc = SVDB_OK;
+      if (t->nq)
+        rc = eng_->search_host(t->q.data(), t->nq, d, d, t->k.data(), t->metric.data(),
+                               cfg_.precision, t->kmax, ids.data(), sc.data());
+      size_t o = 0;
+      for (auto& r : t->reqs) {
+        r->rc = rc;
+        if (rc == SVDB_OK) {
+          r->ids.resize(r->nq * r->k);
+          r->scores.resize(r->nq * r->k);
+          for (size_t i = 0; i < r->nq; i++)
+            for (size_t j = 0; j < r->k; j++) {
+              r->ids[i * r->k + j] = ids[(o + i) * t->kmax + j];
+              r->scores[i * r->k + j] = sc[(o + i) * t->kmax + j];
+            }
+        }
+        o += r->nq;
+      }
+      finish(t->reqs, t0, tick, t->nq);
+    }
+  }
+
+  void run_mutation(Request& r) {
+    if (r.kind == 1) {
+      Vector v;
+      std::memset(&v, 0, sizeof(v));
+      v.dimension = r.dim;
+      v.data = (double*)malloc(r.dim * sizeof(double));
+      std::memcpy(v.data, r.data.data(), r.dim * sizeof(double));
+      size_t idx = vector_db_insert(db_, v);
+      if (idx == (size_t)-1) {
+        free(v.data);
+        r.rc = SVDB_EINVAL;
+      }
+      r.index = idx;
+    } else if (r.kind == 2) {
+      if (r.index >= db_->size) {
+        r.rc = SVDB_EINVAL;
+        return;
+      }
+      Vector v;
+      std::memset(&v, 0, sizeof(v));
+      std::memcpy(v.uuid, db_->vectors[r.index].uuid, UUID_SIZE);
+      v.dimension = r.dim;
+      v.data = (double*)malloc(r.dim * sizeof(double));
+      std::memcpy(v.data, r.data.data(), r.dim * sizeof(double));
+      vector_db_update(db_, r.index, v);
+    } else if (r.kind == 3) {
+      if (r.index >= db_->size) {
+        r.rc = SVDB_EINVAL;
+        return;
+      }
+      vector_db_delete(db_, r.index);
+    }
+  }
+
+  void finish(const std::vector<std::shared_ptr<Request>>& rs, int64_t t0, uint64_t tick,
+              size_t nq) {
+    const int64_t t1 = now_us();
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (auto& r : rs) {
+        r->t_start = t0;
+        r->t_done = t1;
+        r->tick = tick;
+        r->tick_queries = nq;
+        r->done = true;
+      }
+    }
+    cv_done_.notify_all();
+  }
+
+  VectorDatabase* db_;
+  Engine* eng_;
+  svdb_server_config cfg_;
+  std::mutex mu_;
+  std::condition_variable cv_q_, cv_tick_, cv_done_;
+  std::deque<std::shared_ptr<Request>> queue_;
+  std::unique_ptr<Tick> staged_;
+  std::unordered_map<int64_t, std::shared_ptr<Request>> all_;
+  int64_t next_ = 0;
+  uint64_t ticks_ = 0;
+  bool stop_ = false;
+  std::thread stager_, exec_;
+};
+
+}  // namespace svdb
+
+using svdb::Request;
+using svdb::Server;
+
+extern "C" {
+
+void svdb_server_default_config(svdb_server_config* cfg) {
+  if (!cfg) return;
+  cfg->max_batch = 8192;
+  cfg->budget_bytes = (size_t)64 << 30;
+  cfg->precision = SVDB_PREC_AUTO;
+}
+
+svdb_server* svdb_server_start(VectorDatabase* db, const svdb_server_config* cfg) {
+  if (!db || !db->svdb_engine) {
+    svdb::set_error(SVDB_EINVAL, "NULL database");
+    return nullptr;
+  }
+  svdb_server_config c;
+  svdb_server_default_config(&c);
+  if (cfg) c = *cfg;
+  if (c.max_batch == 0 || c.budget_bytes == 0) {
+    svdb::set_error(SVDB_EINVAL, "max_batch and budget_bytes must be positive");
+    return nullptr;
+  }
+  return reinterpret_cast<svdb_server*>(new Server(db, c));
+}
+
+void svdb_server_stop(svdb_server* s) { delete reinterpret_cast<Server*>(s); }
+
+int64_t svdb_server_submit_search(svdb_server* s, const double* queries, size_t nq, size_t dim,
+                                  size_t k, int metric) {
+  Server* sv = reinterpret_cast<Server*>(s);
+  if (!sv || !queries || nq == 0 || k < 1 || k > SVDB_MAX_K || metric < 0 || metric > 2) {
+    svdb::set_error(SVDB_EINVAL, "bad search request");
+    return SVDB_EINVAL;
+  }
+  auto r = std::make_shared<Request>();
+  r->kind = 0;
+  r->nq = nq;
+  r->k = k;
+  r->metric = metric;
+  r->dim = dim;
+  if (sv->dim() != 0 && dim != sv->dim()) {
+    svdb::set_error(SVDB_EINVAL, "query dimension differs from the store's");
+    return SVDB_EINVAL;
+  }
+  r->est_bytes = sv->query_bytes(nq, k);
+  if (r->est_bytes > sv->cfg().budget_bytes || nq > sv->cfg().max_batch) {
+    svdb::set_error(SVDB_ENOMEM, "request larger than the server's batch or memory budget");
+    return SVDB_ENOMEM;
+  }
+  r->data.assign(queries, queries + nq * dim);
+  return sv->submit(r);
+}
+
+int64_t svdb_server_submit_insert(svdb_server* s, const double* row, size_t dim) {
+  Server* sv = reinterpret_cast<Server*>(s);
+  if (!sv || !row || dim == 0) {
+    svdb::set_error(SVDB_EINVAL, "bad insert request");
+    return SVDB_EINVAL;
+  }
+  auto r = std::make_shared<Request>();
+  r->kind = 1;
+  r->dim = dim;
+  r->data.assign(row, row + dim);
+  return sv->submit(r);
+}
+
+int64_t svdb_server_submit_update(svdb_server* s, size_t index, const double* row, size_t dim) {
+  Server* sv = reinterpret_cast<Server*>(s);
+  if (!sv || !row || dim == 0) {
+    svdb::set_error(SVDB_EINVAL, "bad update request");
+    return SVDB_EINVAL;
+  }
+  auto r = std::make_shared<Request>();
+  r->kind = 2;
+  r->index = index;
+  r->dim = dim;
+  r->data.assign(row, row + dim);
+  return sv->submit(r);
+}
+
+int64_t svdb_server_submit_delete(svdb_server* s, size_t index) {
+  Server* sv = reinterpret_cast<Server*>(s);
+  if (!sv) {
+    svdb::set_error(SVDB_EINVAL, "NULL server");
+    return SVDB_EINVAL;
+  }
+  auto r = std::make_shared<Request>();
+  r->kind = 3;
+  r->index = index;
+  return sv->submit(r);
+}
+
+int svdb_server_wait(svdb_server* s, int64_t ticket, size_t* ids, double* scores,
+                     svdb_request_info* info) {
+  Server* sv = reinterpret_cast<Server*>(s);
+  if (!sv) {
+    svdb::set_error(SVDB_EINVAL, "NULL server");
+    return SVDB_EINVAL;
+  }
+  return sv->wait(ticket, ids, scores, info);
+}
+
+}  // extern "C"

@@ -31,7 +31,10 @@ EXPORTS = [
     "svdb_search", "svdb_search_device", "svdb_pair_metrics", "svdb_bulk_insert",
     "svdb_set_shortlist", "svdb_set_budget_bytes", "svdb_enable_shadow", "svdb_get_stats",
     "svdb_set_profiling", "svdb_get_kernel_times", "svdb_last_error", "svdb_device_alloc",
-    "svdb_device_free", "svdb_device_copy", "svdb_device_sync",
+    "svdb_device_free", "svdb_device_copy", "svdb_device_sync", "svdb_server_default_config",
+    "svdb_server_start", "svdb_server_stop", "svdb_server_submit_search",
+    "svdb_server_submit_insert", "svdb_server_submit_update", "svdb_server_submit_delete",
+    "svdb_server_wait",
 ]
 
 
@@ -59,6 +62,15 @@ class Stats(C.Structure):
 
 class KernelTime(C.Structure):
     _fields_ = [("launches", C.c_uint64), ("total_ms", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_size_t), ("budget_bytes", C.c_size_t), ("precision", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("submit_us", C.c_int64), ("start_us", C.c_int64), ("done_us", C.c_int64),
+                ("tick", C.c_uint64), ("tick_queries", C.c_size_t)]
 
 
 _lib = None
@@ -123,6 +135,20 @@ def lib():
     L.svdb_device_copy.restype = C.c_int
     L.svdb_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     L.svdb_device_sync.restype = C.c_int
+    L.svdb_server_default_config.argtypes = [C.POINTER(ServerConfig)]
+    L.svdb_server_start.restype = C.c_void_p
+    L.svdb_server_start.argtypes = [DBP, C.POINTER(ServerConfig)]
+    L.svdb_server_stop.argtypes = [vp]
+    L.svdb_server_submit_search.restype = C.c_int64
+    L.svdb_server_submit_search.argtypes = [vp, vp, sz, sz, sz, C.c_int]
+    L.svdb_server_submit_insert.restype = C.c_int64
+    L.svdb_server_submit_insert.argtypes = [vp, vp, sz]
+    L.svdb_server_submit_update.restype = C.c_int64
+    L.svdb_server_submit_update.argtypes = [vp, sz, vp, sz]
+    L.svdb_server_submit_delete.restype = C.c_int64
+    L.svdb_server_submit_delete.argtypes = [vp, sz]
+    L.svdb_server_wait.restype = C.c_int
+    L.svdb_server_wait.argtypes = [vp, C.c_int64, vp, vp, C.POINTER(RequestInfo)]
     _libc = C.CDLL(None)
     _libc.malloc.restype = C.c_void_p
     _libc.malloc.argtypes = [C.c_size_t]
@@ -374,5 +400,77 @@ class StandaloneKDTree:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """A continuous-batching server over a Database (svdb_server_*)."""
+
+    def __init__(self, db, max_batch=8192, budget_bytes=64 << 30, precision="auto"):
+        cfg = ServerConfig()
+        lib().svdb_server_default_config(C.byref(cfg))
+        cfg.max_batch = max_batch
+        cfg.budget_bytes = budget_bytes
+        cfg.precision = _prec(precision)
+        self.db = db
+        self.p = lib().svdb_server_start(db.p, C.byref(cfg))
+        if not self.p:
+            raise SvdbError(-1, "svdb_server_start")
+        self.pending = {}
+
+    def submit_search(self, Q, k, metric):
+        Q = _f64(np.atleast_2d(Q))
+        t = lib().svdb_server_submit_search(self.p, Q.ctypes.data, Q.shape[0], Q.shape[1], k,
+                                            _metric(metric))
+        if t < 0:
+            raise SvdbError(int(t), "svdb_server_submit_search")
+        self.pending[t] = ("search", Q.shape[0], k)
+        return t
+
+    def submit_insert(self, row):
+        row = _f64(row)
+        t = lib().svdb_server_submit_insert(self.p, row.ctypes.data, row.shape[0])
+        if t < 0:
+            raise SvdbError(int(t), "svdb_server_submit_insert")
+        self.pending[t] = ("insert", 1, 1)
+        return t
+
+    def submit_update(self, index, row):
+        row = _f64(row)
+        t = lib().svdb_server_submit_update(self.p, index, row.ctypes.data, row.shape[0])
+        if t < 0:
+            raise SvdbError(int(t), "svdb_server_submit_update")
+        self.pending[t] = ("update", 0, 0)
+        return t
+
+    def submit_delete(self, index):
+        t = lib().svdb_server_submit_delete(self.p, index)
+        if t < 0:
+            raise SvdbError(int(t), "svdb_server_submit_delete")
+        self.pending[t] = ("delete", 0, 0)
+        return t
+
+    def wait(self, ticket):
+        """-> (rc, ids, scores, info dict) ; ids/scores shaped (nq, k) for searches."""
+        kind, nq, k = self.pending.pop(ticket)
+        ids = np.full((max(nq, 1), max(k, 1)), NOID, dtype=np.uint64)
+        sc = np.full((max(nq, 1), max(k, 1)), np.nan)
+        info = RequestInfo()
+        rc = lib().svdb_server_wait(self.p, ticket, ids.ctypes.data, sc.ctypes.data,
+                                    C.byref(info))
+        inf = {f: getattr(info, f) for f, _ in RequestInfo._fields_}
+        if kind == "search":
+            return rc, ids, sc, inf
+        return rc, (int(ids[0, 0]) if kind == "insert" else None), None, inf
+
+    def stop(self):
+        if getattr(self, "p", None):
+            lib().svdb_server_stop(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.stop()
         except Exception:
             pass

@@ -1,0 +1,157 @@
+"""Serving loop (svdb_server_*): a request answered inside a mixed,
+continuously batched tick gets exactly the ids and FP64 scores it gets when
+run alone on the same store state, and mutations act as barriers in
+submission order.
+
+The requests follow the serving workload's shape: batch sizes log-uniform in
+1..512, k in {1, 10, 100}, metric uniform over the three, about 5% of the
+requests mutations (insert / update / delete).  The reference serves one
+request per connection thread (server.c, compare_handler.c:263-441); the
+"alone" replay below is that one-at-a-time order.
+"""
+import numpy as np
+import pytest
+
+import svdb
+import workloads
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+N, D = 60_000, 128
+
+
+def _requests(rng, n_req, mut_frac=0.05):
+    reqs = []
+    for _ in range(n_req):
+        if rng.random() < mut_frac:
+            reqs.append(("mut", ["insert", "update", "delete"][rng.integers(3)]))
+        else:
+            nq = int(np.exp(rng.uniform(0, np.log(512))))
+            reqs.append(("search", max(1, nq), int(rng.choice([1, 10, 100])),
+                         ["l2", "cos", "dot"][rng.integers(3)]))
+    return reqs
+
+
+def _store(rows):
+    db = svdb.Database(kd_dimension=3)
+    db.bulk_insert(rows)
+    return db
+
+
+def _bits(a):
+    a = np.asarray(a, dtype=np.float64).copy()
+    a[np.isnan(a)] = np.nan
+    return a.view(np.uint64)
+
+
+def test_mixed_ticks_equal_alone():
+    rng = np.random.default_rng(7)
+    rows = workloads.mixture(N, D, seed=41, n_centers=64)
+    pool = workloads.mixture(4096, D, seed=42, n_centers=64)
+    reqs = _requests(rng, 120)
+    # materialise query blocks and mutation payloads once
+    plan = []
+    size = N
+    for r in reqs:
+        if r[0] == "search":
+            _, nq, k, m = r
+            o = int(rng.integers(0, pool.shape[0] - nq + 1))
+            plan.append(("search", pool[o:o + nq].copy(), k, m))
+        else:
+            kind = r[1]
+            if kind == "insert":
+                plan.append(("insert", rng.standard_normal(D)))
+                size += 1
+            elif kind == "update":
+                plan.append(("update", int(rng.integers(size)), rng.standard_normal(D)))
+            else:
+                plan.append(("delete", int(rng.integers(size))))
+                size -= 1
+
+    # mixed: everything submitted at once, then waited for
+    db = _store(rows)
+    srv = svdb.Server(db, max_batch=8192)
+    tickets = []
+    for p in plan:
+        if p[0] == "search":
+            tickets.append(srv.submit_search(p[1], p[2], p[3]))
+        elif p[0] == "insert":
+            tickets.append(srv.submit_insert(p[1]))
+        elif p[0] == "update":
+            tickets.append(srv.submit_update(p[1], p[2]))
+        else:
+            tickets.append(srv.submit_delete(p[1]))
+    mixed = [srv.wait(t) for t in tickets]
+    srv.stop()
+    final_mixed = len(db)
+    db.close()
+
+    # alone: the same requests one at a time through svdb_search
+    db = _store(rows)
+    batched = 0
+    for p, (rc, ids, sc, info) in zip(plan, mixed):
+        assert rc == 0, p[0]
+        if p[0] == "search":
+            Q, k, m = p[1], p[2], p[3]
+            ids_a, sc_a = db.search(Q, k, m, precision="auto")
+            assert np.array_equal(ids, ids_a), (Q.shape, k, m)
+            assert np.array_equal(_bits(sc), _bits(sc_a)), (Q.shape, k, m)
+            batched += info["tick_queries"] > Q.shape[0]
+        elif p[0] == "insert":
+            assert db.insert(p[1]) == ids
+        elif p[0] == "update":
+            db.update(p[1], p[2])
+        else:
+            db.delete(p[1])
+    assert len(db) == final_mixed
+    assert batched > 0  # some requests really shared a tick
+    # the final store agrees with the FP64 oracle
+    Q = pool[:64]
+    ids_a, sc_a = db.search(Q, 10, "l2", precision="auto")
+    X = np.stack([db.read(i)[1] for i in range(len(db))])
+    ids_o, sc_o = O.topk(X, Q, 10, "l2")
+    assert np.array_equal(ids_a, ids_o)
+    assert np.array_equal(_bits(sc_a), _bits(sc_o))
+    db.close()
+
+
+def test_ticks_are_ordered_and_timed():
+    rows = workloads.gaussian(20_000, D, seed=5)
+    db = _store(rows)
+    srv = svdb.Server(db)
+    q = rows[:3] + 1e-3
+    t1 = srv.submit_search(q, 1, "l2")
+    t2 = srv.submit_delete(0)
+    t3 = srv.submit_search(q, 1, "l2")
+    r1, r2, r3 = srv.wait(t1), srv.wait(t2), srv.wait(t3)
+    assert r1[0] == r2[0] == r3[0] == 0
+    assert list(r1[1][:, 0]) == [0, 1, 2]
+    # after deleting row 0 every logical index shifts down by one
+    assert list(r3[1][1:, 0]) == [0, 1]
+    for r in (r1, r2, r3):
+        i = r[3]
+        assert i["submit_us"] <= i["start_us"] <= i["done_us"]
+    assert r1[3]["tick"] < r2[3]["tick"] < r3[3]["tick"]
+    srv.stop()
+    db.close()
+
+
+def test_admission_limits():
+    rows = workloads.gaussian(4096, D, seed=6)
+    db = _store(rows)
+    srv = svdb.Server(db, max_batch=64, budget_bytes=1 << 20)
+    with pytest.raises(svdb.SvdbError):
+        srv.submit_search(rows[:65], 10, "l2")  # more queries than max_batch
+    with pytest.raises(svdb.SvdbError):
+        srv.submit_search(rows[:8], 10, "l2")  # more memory than the budget
+    with pytest.raises(svdb.SvdbError):
+        srv.submit_search(rows[:1, :64], 10, "l2")  # wrong dimension
+    # within both limits: served, and ticks never exceed max_batch queries
+    ts = [srv.submit_search(rows[i:i + 1], 5, "cos") for i in range(40)]
+    out = [srv.wait(t) for t in ts]
+    for i, (rc, ids, sc, info) in enumerate(out):
+        assert rc == 0 and ids[0, 0] == i
+        assert info["tick_queries"] <= 64
+    srv.stop()
+    db.close()
