@@ -72,6 +72,7 @@ def serve(srv, plan, pool, rate=None):
     sched = np.zeros(n, dtype=np.int64)
     lat_us = np.zeros(n, dtype=np.int64)
     tick_q = np.zeros(n, dtype=np.int64)
+    tick_us = np.zeros(n, dtype=np.int64)
     nqs = np.array([p[2] if p[0] == "search" else 0 for p in plan])
     ready = threading.Semaphore(0)
     errors = []
@@ -103,6 +104,7 @@ def serve(srv, plan, pool, rate=None):
                 errors.append((i, rc))
             lat_us[i] = info["done_us"] - sched[i]
             tick_q[i] = info["tick_queries"]
+            tick_us[i] = info["done_us"] - info["start_us"]
 
     ts = time.perf_counter()
     th = [threading.Thread(target=submitter), threading.Thread(target=waiter)]
@@ -119,7 +121,9 @@ def serve(srv, plan, pool, rate=None):
             "queries_s": round(float(nqs.sum()) / el, 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "mean_tick_queries": round(float(tick_q[srch].mean()), 1)}
+            "mean_tick_queries": round(float(tick_q[srch].mean()), 1),
+            "mean_tick_ms": round(float(tick_us[srch].mean()) * 1e-3, 3),
+            "mutation_ms": round(float(tick_us[~srch].mean()) * 1e-3, 3) if (~srch).any() else None}
 
 
 def serving_sweep(db, pool, store_rows, seconds, loads=(0.3, 0.5, 0.7, 0.9)):
@@ -127,7 +131,21 @@ def serving_sweep(db, pool, store_rows, seconds, loads=(0.3, 0.5, 0.7, 0.9)):
     rng = np.random.default_rng(77)
     srv = svdb.Server(db, max_batch=8192, budget_bytes=64 << 30, precision="auto")
     serve(srv, serving_plan(rng, 200, pool.shape[0], store_rows), pool)  # warm
+    db.set_profiling(True)
+    s0 = db.stats()
     sat = serve(srv, serving_plan(rng, 3000, pool.shape[0], store_rows), pool)
+    s1 = db.stats()
+    kt = db.kernel_times()
+    sat["certified"] = s1["certified"] - s0["certified"]
+    sat["fallback_fp64"] = s1["fallbacks"] - s0["fallbacks"]
+    sat["kernel_ms_total"] = {name: round(kt[i][1], 2) for i, name in
+                              ((svdb.K_SCAN_LOWP, "scan_bf16"), (svdb.K_SHORTLIST, "shortlist"),
+                               (svdb.K_RERANK, "rerank"), (svdb.K_SCAN_FP64, "scan_fp64"),
+                               (svdb.K_PREPASS, "sample_prepass"),
+                               (svdb.K_MERGE, "merge"))}
+    sat["kernel_launches"] = {name: int(kt[i][0]) for i, name in
+                              ((svdb.K_SCAN_LOWP, "scan_bf16"), (svdb.K_SCAN_FP64, "scan_fp64"))}
+    db.set_profiling(False)
     log(f"serving saturation: {sat}")
     levels = []
     for f in loads:
@@ -232,7 +250,8 @@ def main():
     achieved = flop_per_launch / (scan_ms * 1e-3) / 1e12
     stage_ms = {name: kt[i][1] / max(1, kt[i][0]) for i, name in
                 ((svdb.K_SCAN_LOWP, "scan_bf16"), (svdb.K_SHORTLIST, "shortlist"),
-                 (svdb.K_RERANK, "rerank"), (svdb.K_SCAN_FP64, "fallback_scan_fp64"))
+                 (svdb.K_RERANK, "rerank"), (svdb.K_SCAN_FP64, "fallback_scan_fp64"),
+                 (svdb.K_PREPASS, "sample_prepass"))
                 if kt[i][0]}
     certified = s1["certified"] - s0["certified"]
     fallbacks = s1["fallbacks"] - s0["fallbacks"]

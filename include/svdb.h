@@ -114,7 +114,7 @@ typedef struct svdb_kernel_time {
     double total_ms;
 } svdb_kernel_time;
 enum { SVDB_K_SCAN_FP64 = 0, SVDB_K_SCAN_LOWP = 1, SVDB_K_SHORTLIST = 2, SVDB_K_RERANK = 3,
-       SVDB_K_MERGE = 4, SVDB_K_COUNT = 5 };
+       SVDB_K_MERGE = 4, SVDB_K_PREPASS = 5 /* sample scan + shortlist */, SVDB_K_COUNT = 6 };
 int svdb_set_profiling(VectorDatabase* db, int on); /* also resets the counters */
 int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_COUNT] */);
 

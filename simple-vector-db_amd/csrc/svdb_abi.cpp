@@ -268,6 +268,59 @@ void vector_db_update(VectorDatabase* db, size_t index, Vector vec) {
   pthread_mutex_unlock(&db->mutex);
 }
 
+}  // extern "C"
+
+// ---- host-mirror halves of the mutations, for the serving loop ----------
+// The serving loop applies a mutation to the device store at once (searches
+// queued behind it must see it) and to the host mirror (db->vectors) later,
+// in the same order, on a thread of its own: a delete shifts the mirror,
+// which for a large store takes far longer than the device update and is
+// not needed to answer searches.
+namespace svdb {
+
+int mirror_reserve(VectorDatabase* db, size_t rows) {
+  pthread_mutex_lock(&db->mutex);
+  const int rc = grow_mirror(db, rows);
+  pthread_mutex_unlock(&db->mutex);
+  return rc ? SVDB_ENOMEM : SVDB_OK;
+}
+
+void mirror_insert(VectorDatabase* db, Vector vec) {
+  pthread_mutex_lock(&db->mutex);
+  if (grow_mirror(db, db->size + 1) == 0)  // reserved by mirror_reserve
+    db->vectors[db->size++] = vec;
+  else
+    free(vec.data);
+  pthread_mutex_unlock(&db->mutex);
+}
+
+void mirror_update(VectorDatabase* db, size_t index, double* data, size_t dim) {
+  pthread_mutex_lock(&db->mutex);
+  if (index < db->size) {
+    free(db->vectors[index].data);
+    db->vectors[index].data = data;
+    db->vectors[index].dimension = dim;
+  } else {
+    free(data);
+  }
+  pthread_mutex_unlock(&db->mutex);
+}
+
+void mirror_delete(VectorDatabase* db, size_t index) {
+  pthread_mutex_lock(&db->mutex);
+  if (index < db->size) {
+    free(db->vectors[index].data);
+    memmove(&db->vectors[index], &db->vectors[index + 1],
+            (db->size - index - 1) * sizeof(Vector));
+    db->size--;
+  }
+  pthread_mutex_unlock(&db->mutex);
+}
+
+}  // namespace svdb
+
+extern "C" {
+
 void vector_db_delete(VectorDatabase* db, size_t index) {
   if (!db) return;
   pthread_mutex_lock(&db->mutex);

@@ -179,6 +179,8 @@ int Engine::ensure_dim(size_t d) {
     sh_.dp = 32 * ks;
     const char* off = std::getenv("SVDB_NO_SHADOW");
     lowp_ = lowp_wanted_ && ks <= 8 && !(off && *off && *off != '0');
+    const char* pp = std::getenv("SVDB_PREPASS");
+    prepass_ = !(pp && *pp == '0');
     return ensure_capacity(init_slots_);
   }
   if (d != d_) {
@@ -583,13 +585,28 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   const uint32_t groups = nq_pad / qpb;
   const uint32_t tiles = nrows_pad / kLowpTile;
   uint32_t ns = std::max<uint32_t>(8, (512 + groups - 1) / groups);
+  // Past k = 32 the lists' own bound (the smallest 8th entry) must stay
+  // well beyond the k-th neighbour: at least 256 lists per query.
+  if (k > 32) ns = std::max<uint32_t>(ns, 64);
   ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, tiles / 4)));
   const uint32_t rps = (uint32_t)round_up((tiles + ns - 1) / ns, 1) * kLowpTile;
   ns = (nrows_pad + rps - 1) / rps;
   // one set of slices per XCD: a multiple of 8 (trailing slices may be empty)
   ns = (uint32_t)round_up(ns, 8);
   const uint32_t NL = ns * kLowpLanesPerQuery;
-  const size_t ncand = (size_t)nq * NL * kLowpLP;
+  // Sample pre-pass: the same scan over ns_pre evenly spaced runs of tp
+  // tiles (at most 1/16 of the store), each lane keeping only the smallest
+  // key of its rows.  The lanes' rows are disjoint, so the L-th smallest of
+  // those minima is the ordering key of a real row with L-1 others at or
+  // below it: the global L-th key lies at or below it, and the main scan's
+  // lists start from it.  They then skip nearly every row from the first
+  // tile on, where otherwise a short slice spends most of its subtiles
+  // filling its lists (the insertion path costs several times the MFMAs).
+  const uint32_t ns_pre = tiles >= 16 * kPrepassSlices ? kPrepassSlices : kPrepassSlices / 4;
+  const bool pre = prepass_ && tiles >= 16 * ns_pre && ns_pre * kLowpLanesPerQuery >= L;
+  const uint32_t tp = pre ? std::min<uint32_t>(kPrepassMaxTiles, tiles / (16 * ns_pre)) : 0;
+  const uint32_t NL_pre = ns_pre * kLowpLanesPerQuery;
+  const size_t ncand = (size_t)nq * std::max(NL * kLowpLP, pre ? NL_pre : 0u);
   if ((rc = scratch(&s_qb_, &s_qb_n_, (size_t)nq_pad * dp * sizeof(uint16_t)))) return rc;
   if ((rc = scratch(&s_qp_, &s_qp_n_, (size_t)nq_pad * sizeof(QParams)))) return rc;
   if ((rc = scratch(&s_ck_, &s_ck_n_, ncand * sizeof(float)))) return rc;
@@ -611,9 +628,24 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   const float* w0 = metric == kL2 ? sh_.nx : metric == kDot ? sh_.xn : sh_.inv;
   const float* w1 = sh_.pen;
   LowpScanArgs sa{sh_.Xb, dp, w0, w1, (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, rps,
-                  nrows_pad, NL, (float*)s_ck_, (uint32_t*)s_cs_};
+                  rps, nrows_pad, NL, (float*)s_ck_, (uint32_t*)s_cs_, nullptr, 0};
+  if (pre) {
+    LowpScanArgs pa = sa;
+    pa.rows_per_slice = tp * kLowpTile;
+    pa.slice_stride = (tiles / ns_pre) * kLowpTile;
+    pa.NL = NL_pre;
+    prof_begin(SVDB_K_PREPASS);
+    HIPCHK(launch_scan_bf16(pa, metric, ns_pre, true, stream_), "k_scan_bf16 (sample)");
+    // thresholds in s_beta_, read by the main scan before the shortlist
+    // below overwrites them
+    HIPCHK(launch_kth_key((const float*)s_ck_, nq, NL_pre, L, (float*)s_beta_, stream_),
+           "k_kth_key");
+    prof_end(SVDB_K_PREPASS);
+    sa.tau0 = (const float*)s_beta_;
+    sa.tau0_stride = 1;
+  }
   prof_begin(SVDB_K_SCAN_LOWP);
-  HIPCHK(launch_scan_bf16(sa, metric, ns, stream_), "k_scan_bf16");
+  HIPCHK(launch_scan_bf16(sa, metric, ns, false, stream_), "k_scan_bf16");
   prof_end(SVDB_K_SCAN_LOWP);
   ShortlistArgs sl{(const float*)s_ck_, (const uint32_t*)s_cs_, nq, NL, L, (uint32_t*)s_ss_,
                    (float*)s_sk_, (float*)s_beta_};

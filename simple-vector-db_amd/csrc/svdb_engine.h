@@ -38,6 +38,14 @@ const char* last_error();
 // into device memory and k_pair_ref32 runs on a process-wide stream.
 int pair_ref32_host(const double* a, const double* b, size_t n, int metric, float* out);
 
+// Host-mirror (db->vectors) halves of insert / update / delete, applied
+// after the device store's half by the serving loop, in mutation order.
+// mirror_reserve grows the mirror's capacity to `rows` ahead of time.
+int mirror_reserve(VectorDatabase* db, size_t rows);
+void mirror_insert(VectorDatabase* db, Vector vec);
+void mirror_update(VectorDatabase* db, size_t index, double* data, size_t dim);
+void mirror_delete(VectorDatabase* db, size_t index);
+
 // Device used by every engine in this process (SVDB_DEVICE, default 0).
 int device_init();
 
@@ -133,6 +141,9 @@ class Engine {
   // low-precision shadow
   bool lowp_ = false;
   bool lowp_wanted_ = true;
+  bool prepass_ = true;  // sample pre-pass before the first-pass scan (SVDB_PREPASS=0: off)
+  static constexpr uint32_t kPrepassSlices = 256;  // 64 on stores of under 16*256 tiles
+  static constexpr uint32_t kPrepassMaxTiles = 8;
   Shadow sh_;
 
   size_t dev_bytes_ = 0;

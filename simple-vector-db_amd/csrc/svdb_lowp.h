@@ -67,10 +67,18 @@ struct LowpScanArgs {
   const QParams* qp;
   uint32_t nq;
   uint32_t rows_per_slice;  // multiple of kLowpTile
+  uint32_t slice_stride;    // rows from one slice's start to the next's (>= rows_per_slice)
   uint32_t nrows_pad;       // rows readable in Xb / aux (multiple of kLowpTile)
   uint32_t NL;              // lists per query = slices * 4
   float* cand_key;          // [nq][NL][LP]
   uint32_t* cand_slot;
+  // Optional per-query starting threshold: query q's lists only take keys
+  // below tau0[q * tau0_stride].  It must be the ordering key of an actual
+  // row that has at least L-1 rows keyed at or below it (e.g. the L-th key
+  // of a shortlist over a sample of the rows), so the global top-L lies
+  // below it; a list left short then reports the threshold as its bound.
+  const float* tau0;
+  uint32_t tau0_stride;
 };
 
 struct ShortlistArgs {
@@ -115,7 +123,13 @@ hipError_t launch_kill_rows(const Shadow& sh, const uint32_t* slots, uint32_t n,
 hipError_t launch_query_prep(const double* Q, size_t ldq, uint32_t nq, uint32_t nq_pad,
                              uint32_t d, size_t dp, int metric, const uint32_t* rho_bits,
                              uint16_t* Qb, QParams* qp, hipStream_t s);
-hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, hipStream_t s);
+// minonly: one entry per list, its smallest key (the sample pre-pass).
+hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, bool minonly,
+                            hipStream_t s);
+// thr[q] = the L-th smallest of the n keys key[q*n .. q*n+n) (+inf when
+// fewer than L are finite).  One wavefront per query.
+hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L, float* thr,
+                          hipStream_t s);
 hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s);
 hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s);
 hipError_t launch_gather_rows(const double* src, size_t lds, const uint32_t* idx, uint32_t n,

@@ -349,6 +349,49 @@ hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ------------------------------------------------ L-th smallest of n keys
+//
+// One wavefront per query; the keys (n <= 64 * kKthPerLane) sit in
+// registers as order-preserving uint32; a 32-step binary search on that
+// order counts, with ballots, how many lie at or below the midpoint.  The
+// result is one of the keys: the L-th smallest.
+
+constexpr int kKthPerLane = 16;
+
+__global__ __launch_bounds__(256) void k_kth_key(const float* __restrict__ key, uint32_t nq,
+                                                 uint32_t n, uint32_t L, float* __restrict__ thr) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= nq) return;
+  uint32_t u[kKthPerLane];
+#pragma unroll
+  for (int i = 0; i < kKthPerLane; i++) {
+    const uint32_t c = i * 64 + lane;
+    u[i] = c < n ? rank_bits32(key[(size_t)q * n + c]) : 0xFFFFFFFFu;
+  }
+  const uint32_t inf_bits = rank_bits32(INFINITY);
+  uint32_t lo = 0, hi = 0xFFFFFFFFu;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < kKthPerLane; i++) cnt += __popcll(__ballot(u[i] <= mid));
+    if (cnt >= L) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lane == 0)
+    thr[q] = (lo >= inf_bits || n < L) ? INFINITY
+                                       : __uint_as_float((lo >> 31) ? (lo & 0x7FFFFFFFu) : ~lo);
+}
+
+hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L, float* thr,
+                          hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  if (n > 64u * kKthPerLane || L == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_kth_key, dim3((nq + 3) / 4), dim3(256), 0, s, key, nq, n, L, thr);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------- FP64 rerank
 //
 // One wavefront per query.  Lanes take the candidates 64 at a time (the

@@ -66,7 +66,10 @@ __device__ __forceinline__ void issue_tile(const LowpScanArgs& a, unsigned char*
     load_lds4(a.w1 + row0 + (wave - 2) * 64 + lane, aux + kLowpTile + (wave - 2) * 64);
 }
 
-template <int METRIC, int KS, int QF>
+// MINONLY: each lane keeps only the smallest key of its rows (one list
+// entry, cand_key[q * NL + list]); the sample pre-pass uses it to bound the
+// L-th key cheaply.
+template <int METRIC, int KS, int QF, bool MINONLY>
 __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   using G = TileGeom<KS>;
   __shared__ __attribute__((aligned(16))) unsigned char buf0[G::BUF_B];
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   const uint32_t xcd = blockIdx.x % 8, local = blockIdx.x / 8;
   const uint32_t qbase = (local % nqb) * qpb + wave * QF * 16;
   const uint32_t slice = (local / nqb) * 8 + xcd;
-  const uint32_t r0 = slice * a.rows_per_slice;
+  const uint32_t r0 = slice * a.slice_stride;
   const uint32_t r1 = min(r0 + a.rows_per_slice, a.nrows_pad);
   const uint32_t ntiles = r1 > r0 ? (r1 - r0) / kLowpTile : 0;
   const bool active = qbase < a.nq;  // wave-uniform
@@ -102,10 +105,20 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) B[f][ks] = *reinterpret_cast<const bf16x8*>(qrow + ks * 32);
     P[f] = a.qp[min(q, a.nq - 1)].P;
-    tau[f] = INFINITY;
+    // Padding lanes (q >= nq) never insert; real ones start from the
+    // optional threshold (a NaN threshold counts as none).
+    float t0 = -INFINITY;
+    if (q < a.nq) {
+      t0 = INFINITY;
+      if (a.tau0) {
+        const float t = a.tau0[(size_t)q * a.tau0_stride];
+        if (t < INFINITY) t0 = t;
+      }
+    }
+    tau[f] = MINONLY ? INFINITY : t0;
 #pragma unroll
     for (int j = 0; j < LP; j++) {
-      lk[f][j] = INFINITY;
+      lk[f][j] = t0;
       ls[f][j] = ~0u;
     }
   }
@@ -126,7 +139,9 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
       for (int r = 0; r < 4; r++)
         k4[r] = COS ? fmaf(acc[f][r], wv[r], pv[r]) : fmaf(P[f], wv[r], acc[f][r]);
       const float mn = fminf(fminf(k4[0], k4[1]), fminf(k4[2], k4[3]));
-      if (mn < tau[f]) {
+      if constexpr (MINONLY) {
+        tau[f] = fminf(tau[f], mn);
+      } else if (mn < tau[f]) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           if (k4[r] < lk[f][LP - 1]) {
@@ -205,6 +220,10 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   for (int f = 0; f < QF; f++) {
     const uint32_t q = qbase + f * 16 + ql;
     if (q >= a.nq) continue;
+    if constexpr (MINONLY) {
+      a.cand_key[(size_t)q * a.NL + list] = tau[f];
+      continue;
+    }
     const size_t o = ((size_t)q * a.NL + list) * LP;
 #pragma unroll
     for (int j = 0; j < LP; j++) {
@@ -215,8 +234,9 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
 }
 
 template <int METRIC, int KS, int QF>
-static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
-  auto fn = k_scan_bf16<METRIC, KS, QF>;
+static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, bool minonly,
+                              hipStream_t s) {
+  auto fn = minonly ? k_scan_bf16<METRIC, KS, QF, true> : k_scan_bf16<METRIC, KS, QF, false>;
   constexpr size_t shm = 0;  // both tile buffers are static LDS
   const uint32_t qpb = kLowpWaves * QF * 16;
   // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid.
@@ -227,24 +247,26 @@ static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, hipStream
 }
 
 template <int METRIC>
-static hipError_t scan_bf16_m(const LowpScanArgs& a, uint32_t nslices, hipStream_t s) {
+static hipError_t scan_bf16_m(const LowpScanArgs& a, uint32_t nslices, bool minonly,
+                              hipStream_t s) {
   switch (a.dp / 32) {
-    case 1: return scan_bf16_t<METRIC, 1, 4>(a, nslices, s);
-    case 2: return scan_bf16_t<METRIC, 2, 4>(a, nslices, s);
-    case 4: return scan_bf16_t<METRIC, 4, 4>(a, nslices, s);
-    case 8: return scan_bf16_t<METRIC, 8, 2>(a, nslices, s);
+    case 1: return scan_bf16_t<METRIC, 1, 4>(a, nslices, minonly, s);
+    case 2: return scan_bf16_t<METRIC, 2, 4>(a, nslices, minonly, s);
+    case 4: return scan_bf16_t<METRIC, 4, 4>(a, nslices, minonly, s);
+    case 8: return scan_bf16_t<METRIC, 8, 2>(a, nslices, minonly, s);
   }
   return hipErrorInvalidValue;
 }
 
 int lowp_queries_per_block(size_t dp) { return dp / 32 == 8 ? kLowpWaves * 2 * 16 : kLowpQPB; }
 
-hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, hipStream_t s) {
+hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, bool minonly,
+                            hipStream_t s) {
   if (a.nq == 0 || nslices == 0) return hipSuccess;
   switch (metric) {
-    case kL2: return scan_bf16_m<kL2>(a, nslices, s);
-    case kCos: return scan_bf16_m<kCos>(a, nslices, s);
-    case kDot: return scan_bf16_m<kDot>(a, nslices, s);
+    case kL2: return scan_bf16_m<kL2>(a, nslices, minonly, s);
+    case kCos: return scan_bf16_m<kCos>(a, nslices, minonly, s);
+    case kDot: return scan_bf16_m<kDot>(a, nslices, minonly, s);
   }
   return hipErrorInvalidValue;
 }

@@ -9,7 +9,9 @@
 // while a tick is on the GPU wait for the next tick; the queries of the
 // next tick are staged (copied and grouped on the host) by a second thread
 // while the current tick runs.  A mutation is a barrier: searches queued
-// before it see the store without it, searches after it see it.
+// before it see the store without it, searches after it see it.  The host
+// mirror (db->vectors) follows the device store on a third thread, in the
+// same order; a mutation completes once both halves are applied.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -65,6 +67,7 @@ class Server {
     eng_ = static_cast<Engine*>(db->svdb_engine);
     stager_ = std::thread([this] { stage_loop(); });
     exec_ = std::thread([this] { exec_loop(); });
+    mirror_ = std::thread([this] { mirror_loop(); });
   }
 
   ~Server() {
@@ -76,6 +79,12 @@ class Server {
     cv_tick_.notify_all();
     stager_.join();
     exec_.join();
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      mirror_stop_ = true;  // after the executor: no more mirror work comes
+    }
+    cv_mirror_.notify_all();
+    mirror_.join();
     // requests still queued are failed, not dropped
     std::lock_guard<std::mutex> g(mu_);
     for (auto& r : queue_) {
@@ -207,8 +216,21 @@ class Server {
       const int64_t t0 = now_us();
       const uint64_t tick = ++ticks_;
       if (!t->mutations.empty()) {
-        for (auto& m : t->mutations) run_mutation(*m);
-        finish(t->mutations, t0, tick, 0);
+        // Device halves now (the next tick's searches must see them); host
+        // mirror halves, in the same order, on the mirror thread, which
+        // completes the requests.
+        for (auto& m : t->mutations) {
+          run_mutation(*m);
+          if (m->rc != SVDB_OK) {
+            finish({m}, t0, tick, 0);
+          } else {
+            std::lock_guard<std::mutex> g(mu_);
+            m->t_start = t0;
+            m->tick = tick;
+            mirror_q_.push_back(m);
+          }
+        }
+        cv_mirror_.notify_all();
         continue;
       }
       const size_t d = eng_->dim();
@@ -236,42 +258,56 @@ class Server {
     }
   }
 
+  // Device half of a mutation (the engine store and its shadow).  Indices
+  // are logical positions at this point of the mutation order, which the
+  // engine tracks itself (the host mirror may lag behind).
   void run_mutation(Request& r) {
+    const size_t rows = eng_->rows();
     if (r.kind == 1) {
-      Vector v;
-      std::memset(&v, 0, sizeof(v));
-      v.dimension = r.dim;
-      v.data = (double*)malloc(r.dim * sizeof(double));
-      std::memcpy(v.data, r.data.data(), r.dim * sizeof(double));
-      size_t idx = vector_db_insert(db_, v);
-      if (idx == (size_t)-1) {
-        free(v.data);
-        r.rc = SVDB_EINVAL;
-      }
-      r.index = idx;
+      r.index = rows;
+      if ((r.rc = mirror_reserve(db_, rows + 1)) == SVDB_OK)
+        r.rc = eng_->append(r.data.data(), 1, r.dim, r.dim, nullptr);
+    } else if (r.index >= rows) {
+      set_error(SVDB_EINVAL, "index out of range");
+      r.rc = SVDB_EINVAL;
     } else if (r.kind == 2) {
-      if (r.index >= db_->size) {
-        r.rc = SVDB_EINVAL;
-        return;
-      }
-      Vector v;
-      std::memset(&v, 0, sizeof(v));
-      std::memcpy(v.uuid, db_->vectors[r.index].uuid, UUID_SIZE);
-      v.dimension = r.dim;
-      v.data = (double*)malloc(r.dim * sizeof(double));
-      std::memcpy(v.data, r.data.data(), r.dim * sizeof(double));
-      vector_db_update(db_, r.index, v);
-    } else if (r.kind == 3) {
-      if (r.index >= db_->size) {
-        r.rc = SVDB_EINVAL;
-        return;
-      }
-      vector_db_delete(db_, r.index);
+      r.rc = eng_->overwrite(r.index, r.data.data(), r.dim);
+    } else {
+      r.rc = eng_->erase(r.index);
     }
   }
 
-  void finish(const std::vector<std::shared_ptr<Request>>& rs, int64_t t0, uint64_t tick,
-              size_t nq) {
+  // Host-mirror half, in mutation order; completes the request.
+  void mirror_loop() {
+    for (;;) {
+      std::shared_ptr<Request> r;
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_mirror_.wait(g, [&] { return mirror_stop_ || !mirror_q_.empty(); });
+        if (mirror_q_.empty()) return;
+        r = mirror_q_.front();
+        mirror_q_.pop_front();
+      }
+      if (r->kind == 1 || r->kind == 2) {
+        double* data = (double*)malloc(r->dim * sizeof(double));
+        if (data) std::memcpy(data, r->data.data(), r->dim * sizeof(double));
+        if (r->kind == 1) {
+          Vector v;
+          std::memset(&v, 0, sizeof(v));
+          v.dimension = r->dim;
+          v.data = data;
+          mirror_insert(db_, v);
+        } else {
+          mirror_update(db_, r->index, data, r->dim);
+        }
+      } else {
+        mirror_delete(db_, r->index);
+      }
+      finish({r}, r->t_start, r->tick, 0);
+    }
+  }
+
+  void finish(std::vector<std::shared_ptr<Request>> rs, int64_t t0, uint64_t tick, size_t nq) {
     const int64_t t1 = now_us();
     {
       std::lock_guard<std::mutex> g(mu_);
@@ -290,14 +326,16 @@ class Server {
   Engine* eng_;
   svdb_server_config cfg_;
   std::mutex mu_;
-  std::condition_variable cv_q_, cv_tick_, cv_done_;
+  std::condition_variable cv_q_, cv_tick_, cv_done_, cv_mirror_;
   std::deque<std::shared_ptr<Request>> queue_;
+  std::deque<std::shared_ptr<Request>> mirror_q_;
+  bool mirror_stop_ = false;
   std::unique_ptr<Tick> staged_;
   std::unordered_map<int64_t, std::shared_ptr<Request>> all_;
   int64_t next_ = 0;
   uint64_t ticks_ = 0;
   bool stop_ = false;
-  std::thread stager_, exec_;
+  std::thread stager_, exec_, mirror_;
 };
 
 }  // namespace svdb
