@@ -29,6 +29,7 @@ import numpy as np  # noqa: E402
 METRIC = "queries/s + p50/p99 ms, 10M×128 top-10, ids exact vs FP64 oracle"
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 PEAK_FP64_TFLOPS = 78.6
+PROFILED_TRAFFIC = 2_680_094_720 + 67_108_864  # fetch + write bytes, one k_scan_bf16 launch
 
 
 def log(*a):
@@ -372,7 +373,10 @@ def main():
             "peak": PEAK_BF16_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
-            "traffic": None,
+            # HBM bytes per launch of this kernel at this workload, measured with
+            # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a plain copy kernel
+            # (profiles/k_scan_bf16_mixture_10Mx128_b16384.md); None at other sizes
+            "traffic": PROFILED_TRAFFIC if (args.rows, B, d) == (10_000_000, 16384, 128) else None,
             "flop_per_launch": flop_per_launch,
             "avg_launch_ms": round(scan_ms, 4),
         },

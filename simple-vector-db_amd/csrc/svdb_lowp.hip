@@ -305,23 +305,36 @@ __global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
   uint64_t tk = ~0ull;
   uint32_t ts = ~0u;
   float lane_min = INFINITY;
-  for (uint32_t c0 = 0; c0 < n; c0 += 64) {
-    const uint32_t c = c0 + lane;
-    float key = INFINITY;
-    uint32_t slot = ~0u;
-    if (c < n) {
-      key = a.cand_key[base + c];
-      slot = a.cand_slot[base + c];
-      if ((c % kLowpLP) == kLowpLP - 1) lane_min = fminf(lane_min, key);
+  // Entries are read kSlU batches of 64 at a time, so the loads of a batch
+  // group are in flight together (most entries fail the ballot and cost
+  // only their load).
+  constexpr int kSlU = 8;
+  for (uint32_t c0 = 0; c0 < n; c0 += 64 * kSlU) {
+    float key[kSlU];
+    uint32_t slot[kSlU];
+#pragma unroll
+    for (int u = 0; u < kSlU; u++) {
+      const uint32_t c = c0 + u * 64 + lane;
+      key[u] = INFINITY;
+      slot[u] = ~0u;
+      if (c < n) {
+        key[u] = a.cand_key[base + c];
+        slot[u] = a.cand_slot[base + c];
+      }
     }
-    const uint64_t K = slot == ~0u ? ~0ull : (uint64_t)rank_bits32(key);
-    uint64_t m = __ballot(slot != ~0u && lt2(K, slot, tk, ts));
-    while (m) {
-      int src = __ffsll((long long)m) - 1;
-      m &= m - 1;
-      L.insert(uni64(__shfl(K, src)), uni32(__shfl(slot, src)), kk, lane);
+#pragma unroll
+    for (int u = 0; u < kSlU; u++) {
+      const uint32_t c = c0 + u * 64 + lane;
+      if (c < n && (c % kLowpLP) == kLowpLP - 1) lane_min = fminf(lane_min, key[u]);
+      const uint64_t K = slot[u] == ~0u ? ~0ull : (uint64_t)rank_bits32(key[u]);
+      uint64_t m = __ballot(slot[u] != ~0u && lt2(K, slot[u], tk, ts));
+      while (m) {
+        int src = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        L.insert(uni64(__shfl(K, src)), uni32(__shfl(slot[u], src)), kk, lane);
+      }
+      L.at(kk - 1, tk, ts);
     }
-    L.at(kk - 1, tk, ts);
   }
   // min over lanes of the lists' 8th entries
   for (int off = 32; off >= 1; off >>= 1) lane_min = fminf(lane_min, __shfl_xor(lane_min, off));
