@@ -45,7 +45,8 @@ enum { SVDB_METRIC_L2 = 0, SVDB_METRIC_COS = 1, SVDB_METRIC_DOT = 2 };
 enum {
     SVDB_PREC_FP64 = 0, /* plain FP64 exhaustive scan */
     SVDB_PREC_BF16 = 1, /* BF16 MFMA first pass + shortlist + FP64 rerank */
-    SVDB_PREC_FP8 = 2,  /* FP8-e4m3 MFMA first pass + shortlist + FP64 rerank */
+    SVDB_PREC_FP8 = 2,  /* FP8-e4m3 MFMA first pass + shortlist + FP64 rerank
+                           (needs svdb_enable_shadow(db, SVDB_PREC_FP8); d <= 1536) */
     SVDB_PREC_AUTO = 3  /* BF16 when the store has a shadow, else FP64 */
 };
 
@@ -92,7 +93,8 @@ int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t di
 /* Engine knobs. */
 int svdb_set_shortlist(VectorDatabase* db, size_t width); /* first-pass shortlist L (>= k) */
 int svdb_set_budget_bytes(VectorDatabase* db, size_t bytes); /* device-memory budget */
-int svdb_enable_shadow(VectorDatabase* db, int precision); /* build BF16/FP8 shadow */
+int svdb_enable_shadow(VectorDatabase* db, int precision); /* build the BF16 (d <= 256) or
+                                                               FP8 (d <= 1536) shadow */
 
 typedef struct svdb_stats {
     size_t rows;            /* logical rows */
@@ -105,6 +107,7 @@ typedef struct svdb_stats {
     uint64_t certified;     /* low-precision queries certified exact */
     uint64_t fallbacks;     /* low-precision queries rescanned in FP64 */
     double last_max_rel_err;/* max |first-pass score - FP64 score| / |FP64| of the last call */
+    uint64_t widened;       /* of the certified: certified only by the 1024-wide retry */
 } svdb_stats;
 int svdb_get_stats(VectorDatabase* db, svdb_stats* out);
 

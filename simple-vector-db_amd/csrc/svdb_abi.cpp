@@ -613,6 +613,7 @@ int svdb_get_stats(VectorDatabase* db, svdb_stats* out) {
   out->certified = s.certified;
   out->fallbacks = s.fallbacks;
   out->last_max_rel_err = s.last_max_rel_err;
+  out->widened = s.widened;
   return SVDB_OK;
 }
 

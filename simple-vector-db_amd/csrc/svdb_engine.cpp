@@ -116,8 +116,9 @@ Engine::~Engine() {
   (void)hipStreamSynchronize(stream_);
   void* bufs[] = {X_, nx2_, xn_, alive_, rank_, s_q_, s_qn_, s_k_, s_pk_, s_ps_, s_ids_, s_sc_,
                   s_pair_, s_pout_, s_qb_, s_qp_, s_ck_, s_cs_, s_sk_, s_ss_, s_beta_, s_fail_,
-                  s_err_, s_fq_, s_fid_, s_fsc_, s_fk_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
-                  sh_.rho_bits /* degen_count and degen live inside this block */};
+                  s_err_, s_fq_, s_fid_, s_fsc_, s_fk_, s_sk2_, s_ss2_, s_beta2_, s_fail2_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
+                  sh_.rho_bits /* degen_count and degen live inside this block */, sh8_.X8,
+                  sh8_.sx, sh8_.nx, sh8_.xn, sh8_.inv, sh8_.pen, sh8_.rho_bits};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h_fail_) (void)hipHostFree(h_fail_);
@@ -179,6 +180,12 @@ int Engine::ensure_dim(size_t d) {
     sh_.dp = 32 * ks;
     const char* off = std::getenv("SVDB_NO_SHADOW");
     lowp_ = lowp_wanted_ && ks <= 8 && !(off && *off && *off != '0');
+    // FP8 rows: the smallest supported padded length (0: no FP8 first pass)
+    for (size_t c : {256, 512, 768, 1024, 1536})
+      if (d <= c) {
+        sh8_.dp = c;
+        break;
+      }
     const char* pp = std::getenv("SVDB_PREPASS");
     prepass_ = !(pp && *pp == '0');
     return ensure_capacity(init_slots_);
@@ -201,6 +208,10 @@ int Engine::ensure_capacity(size_t need) {
   nc = std::min(nc, (size_t)UINT32_MAX / 128 * 128 - 128);
   if (lowp_) {
     int rs = alloc_shadow(nc);
+    if (rs) return rs;
+  }
+  if (lowp8_) {
+    int rs = alloc_shadow8(nc);
     if (rs) return rs;
   }
   const size_t nw = nc / 64, ow = cap_ / 64;
@@ -289,32 +300,99 @@ int Engine::alloc_shadow(size_t nc) {
 
 // Quantizes slots [first, first+n) (or the device slot list) into the shadow.
 int Engine::shadow_rows(uint32_t first, uint32_t n, const uint32_t* dslots) {
-  if (!lowp_ || n == 0) return SVDB_OK;
-  HIPCHK(launch_quantize_rows(sh_, X_, ld_, nx2_, xn_, dslots, first, n, (uint32_t)d_, stream_),
-         "k_quantize_rows");
+  if (n == 0) return SVDB_OK;
+  if (lowp_)
+    HIPCHK(launch_quantize_rows(sh_, X_, ld_, nx2_, xn_, dslots, first, n, (uint32_t)d_, stream_),
+           "k_quantize_rows");
+  if (lowp8_)
+    HIPCHK(launch_quantize_rows8(sh8_, X_, ld_, nx2_, xn_, dslots, first, n, (uint32_t)d_, stream_),
+           "k_quantize_rows8");
+  return SVDB_OK;
+}
+
+// (Re)allocates the FP8 shadow for `nc` slots, as alloc_shadow does the BF16 one.
+int Engine::alloc_shadow8(size_t nc) {
+  Shadow8 n = sh8_;
+  int rc;
+  const size_t xb = nc * sh8_.dp, fb = nc * sizeof(float);
+  float** arrs[5] = {&n.sx, &n.nx, &n.xn, &n.inv, &n.pen};
+  if ((rc = dev_alloc((void**)&n.X8, xb))) return rc;
+  for (int i = 0; i < 5; i++)
+    if ((rc = dev_alloc((void**)arrs[i], fb))) {
+      dev_free(n.X8, xb);
+      for (int j = 0; j < i; j++) dev_free(*arrs[j], fb);
+      return rc;
+    }
+  if (!sh8_.rho_bits) {
+    if ((rc = dev_alloc((void**)&n.rho_bits, 3 * sizeof(uint32_t) + kDegenCap * sizeof(uint32_t))))
+      return rc;
+    n.degen_count = n.rho_bits + 1;
+    n.degen = n.rho_bits + 3;
+    HIPCHK(hipMemsetAsync(n.rho_bits, 0, 3 * sizeof(uint32_t), stream_), "memset");
+  }
+  HIPCHK(hipMemsetAsync(n.X8, 0, xb, stream_), "memset");
+  HIPCHK(launch_shadow_init8(n, 0, (uint32_t)nc, stream_), "k_shadow_init8");
+  if (sh8_.X8 && nphys_) {
+    HIPCHK(hipMemcpyAsync(n.X8, sh8_.X8, nphys_ * sh8_.dp, hipMemcpyDeviceToDevice, stream_), "grow");
+    const float* olds[5] = {sh8_.sx, sh8_.nx, sh8_.xn, sh8_.inv, sh8_.pen};
+    for (int i = 0; i < 5; i++)
+      HIPCHK(hipMemcpyAsync(*arrs[i], olds[i], nphys_ * sizeof(float), hipMemcpyDeviceToDevice,
+                            stream_), "grow");
+  }
+  HIPCHK(hipStreamSynchronize(stream_), "grow");
+  if (sh8_.X8) {
+    const size_t ofb = cap_ * sizeof(float);
+    dev_free(sh8_.X8, cap_ * sh8_.dp);
+    float* olds[5] = {sh8_.sx, sh8_.nx, sh8_.xn, sh8_.inv, sh8_.pen};
+    for (int i = 0; i < 5; i++) dev_free(olds[i], ofb);
+  }
+  sh8_ = n;
   return SVDB_OK;
 }
 
 int Engine::enable_shadow(int precision) {
   std::lock_guard<std::mutex> g(mu_);
-  if (precision != SVDB_PREC_BF16 && precision != SVDB_PREC_AUTO) {
-    set_error(SVDB_EINVAL, "only the BF16 shadow is available");
+  if (precision != SVDB_PREC_BF16 && precision != SVDB_PREC_FP8 && precision != SVDB_PREC_AUTO) {
+    set_error(SVDB_EINVAL, "enable_shadow takes SVDB_PREC_BF16, SVDB_PREC_FP8 or SVDB_PREC_AUTO");
     return SVDB_EINVAL;
   }
-  lowp_wanted_ = true;
-  if (lowp_ || d_ == 0) return SVDB_OK;
-  if (sh_.dp > 256) {
+  const bool fp8 = precision == SVDB_PREC_FP8;
+  if (!fp8) lowp_wanted_ = true;
+  if (d_ == 0) {
+    if (fp8) {
+      set_error(SVDB_EINVAL, "the FP8 shadow is built once the store has rows");
+      return SVDB_EINVAL;
+    }
+    return SVDB_OK;
+  }
+  if (fp8 ? lowp8_ : lowp_) return SVDB_OK;
+  if (!fp8 && sh_.dp > 256) {
     set_error(SVDB_EINVAL, "BF16 first pass supports dimensions up to 256");
     return SVDB_EINVAL;
   }
-  int rc = alloc_shadow(cap_);
+  if (fp8 && !fp8_dp_supported(sh8_.dp)) {
+    set_error(SVDB_EINVAL, "FP8 first pass supports dimensions up to 1536");
+    return SVDB_EINVAL;
+  }
+  int rc = fp8 ? alloc_shadow8(cap_) : alloc_shadow(cap_);
   if (rc) return rc;
-  lowp_ = true;
-  if ((rc = shadow_rows(0, (uint32_t)nphys_, nullptr))) return rc;
+  // quantize the existing rows into this shadow only
+  const bool had_b = lowp_, had_8 = lowp8_;
+  lowp_ = !fp8;
+  lowp8_ = fp8;
+  rc = shadow_rows(0, (uint32_t)nphys_, nullptr);
+  lowp_ = had_b || !fp8;
+  lowp8_ = had_8 || fp8;
+  if (rc) return rc;
   for (size_t s = 0; s < nphys_; s++)
     if (!((h_alive_[s >> 6] >> (s & 63)) & 1ull)) {
-      HIPCHK(hipMemsetAsync(sh_.Xb + s * sh_.dp, 0, sh_.dp * sizeof(uint16_t), stream_), "memset");
-      HIPCHK(launch_shadow_init(sh_, (uint32_t)s, 1, stream_), "k_shadow_init");
+      if (fp8) {
+        HIPCHK(hipMemsetAsync(sh8_.X8 + s * sh8_.dp, 0, sh8_.dp, stream_), "memset");
+        HIPCHK(launch_shadow_init8(sh8_, (uint32_t)s, 1, stream_), "k_shadow_init8");
+      } else {
+        HIPCHK(hipMemsetAsync(sh_.Xb + s * sh_.dp, 0, sh_.dp * sizeof(uint16_t), stream_), "memset");
+        HIPCHK(launch_shadow_init(sh_, (uint32_t)s, 1, stream_), "k_shadow_init");
+      }
     }
   HIPCHK(hipStreamSynchronize(stream_), "enable_shadow");
   return SVDB_OK;
@@ -434,12 +512,16 @@ int Engine::erase(size_t logical) {
   }
   const uint32_t s = select_slot(logical);
   const size_t w = s >> 6, nw = (nphys_ + 63) / 64;
+  // the first pass must never pick a tombstone
   if (lowp_) {
-    // the first pass must never pick a tombstone
     HIPCHK(hipMemsetAsync(sh_.Xb + (size_t)s * sh_.dp, 0, sh_.dp * sizeof(uint16_t), stream_), "memset");
     HIPCHK(launch_shadow_init(sh_, s, 1, stream_), "k_shadow_init");
-    HIPCHK(hipStreamSynchronize(stream_), "erase");
   }
+  if (lowp8_) {
+    HIPCHK(hipMemsetAsync(sh8_.X8 + (size_t)s * sh8_.dp, 0, sh8_.dp, stream_), "memset");
+    HIPCHK(launch_shadow_init8(sh8_, s, 1, stream_), "k_shadow_init8");
+  }
+  if (lowp_ || lowp8_) HIPCHK(hipStreamSynchronize(stream_), "erase");
   h_alive_[w] &= ~(1ull << (s & 63));
   for (size_t v = w + 1; v < nw; v++) h_rank_[v]--;
   nalive_--;
@@ -568,10 +650,11 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
 // (all columns, one metric and k).  Queries the rerank cannot certify are
 // gathered and answered by the FP64 scan.
 int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k,
-                     uint64_t* dids, double* dscores, uint32_t ostride) {
+                     uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
   int rc;
-  const size_t dp = sh_.dp;
-  const uint32_t qpb = (uint32_t)lowp_queries_per_block(dp);
+  const size_t dp = fp8 ? sh8_.dp : sh_.dp;
+  const size_t qbytes = fp8 ? 1 : sizeof(uint16_t);  // bytes per query element
+  const uint32_t qpb = (uint32_t)(fp8 ? fp8_queries_per_block(dp) : lowp_queries_per_block(dp));
   const uint32_t nq_pad = (uint32_t)round_up(nq, qpb);
   // Shortlist: the configured width (64 by default) for small k; 256 once
   // k passes 32, where the k-th and L-th neighbours need more room than the
@@ -588,6 +671,9 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   // Past k = 32 the lists' own bound (the smallest 8th entry) must stay
   // well beyond the k-th neighbour: at least 256 lists per query.
   if (k > 32) ns = std::max<uint32_t>(ns, 64);
+  // FP8: the widening retry takes up to kMaxShort candidates from these
+  // lists, so keep at least 512 of them per query.
+  if (fp8) ns = std::max<uint32_t>(ns, 128);
   ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, tiles / 4)));
   const uint32_t rps = (uint32_t)round_up((tiles + ns - 1) / ns, 1) * kLowpTile;
   ns = (nrows_pad + rps - 1) / rps;
@@ -607,7 +693,7 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   const uint32_t tp = pre ? std::min<uint32_t>(kPrepassMaxTiles, tiles / (16 * ns_pre)) : 0;
   const uint32_t NL_pre = ns_pre * kLowpLanesPerQuery;
   const size_t ncand = (size_t)nq * std::max(NL * kLowpLP, pre ? NL_pre : 0u);
-  if ((rc = scratch(&s_qb_, &s_qb_n_, (size_t)nq_pad * dp * sizeof(uint16_t)))) return rc;
+  if ((rc = scratch(&s_qb_, &s_qb_n_, (size_t)nq_pad * dp * qbytes))) return rc;
   if ((rc = scratch(&s_qp_, &s_qp_n_, (size_t)nq_pad * sizeof(QParams)))) return rc;
   if ((rc = scratch(&s_ck_, &s_ck_n_, ncand * sizeof(float)))) return rc;
   if ((rc = scratch(&s_cs_, &s_cs_n_, ncand * sizeof(uint32_t)))) return rc;
@@ -620,22 +706,36 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   if ((rc = sync_alive())) return rc;
 
   uint32_t* fail_count = (uint32_t*)s_fail_;
-  uint32_t* fail_list = fail_count + 1;
+  uint32_t* fail_list = fail_count + 1;  // may be redirected to the widening retry's list
   HIPCHK(hipMemsetAsync(fail_count, 0, sizeof(uint32_t), stream_), "memset");
   HIPCHK(hipMemsetAsync(s_err_, 0, (size_t)nq * sizeof(double), stream_), "memset");
-  HIPCHK(launch_query_prep(dQ, ldq, nq, nq_pad, (uint32_t)d_, dp, metric, sh_.rho_bits,
-                           (uint16_t*)s_qb_, (QParams*)s_qp_, stream_), "k_query_prep");
-  const float* w0 = metric == kL2 ? sh_.nx : metric == kDot ? sh_.xn : sh_.inv;
-  const float* w1 = sh_.pen;
-  LowpScanArgs sa{sh_.Xb, dp, w0, w1, (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, rps,
-                  rps, nrows_pad, NL, (float*)s_ck_, (uint32_t*)s_cs_, nullptr, 0};
+  if (fp8)
+    HIPCHK(launch_query_prep8(dQ, ldq, nq, nq_pad, (uint32_t)d_, dp, metric, sh8_.rho_bits,
+                              (uint8_t*)s_qb_, (QParams*)s_qp_, stream_), "k_query_prep8");
+  else
+    HIPCHK(launch_query_prep(dQ, ldq, nq, nq_pad, (uint32_t)d_, dp, metric, sh_.rho_bits,
+                             (uint16_t*)s_qb_, (QParams*)s_qp_, stream_), "k_query_prep");
+  // per-row ordering-key arrays: BF16 L2 (nx, pen), dot (xn, pen), cosine
+  // (inv, pen); FP8 L2 (nx, sx), dot (xn, sx), cosine (sx/‖x‖, pen)
+  const float* w0 = fp8 ? (metric == kL2 ? sh8_.nx : metric == kDot ? sh8_.xn : sh8_.inv)
+                        : (metric == kL2 ? sh_.nx : metric == kDot ? sh_.xn : sh_.inv);
+  const float* w1 = fp8 ? (metric == kCos ? sh8_.pen : sh8_.sx) : sh_.pen;
+  const uint32_t* degen = fp8 ? sh8_.degen : sh_.degen;
+  const uint32_t* degen_count = fp8 ? sh8_.degen_count : sh_.degen_count;
+  auto scan = [&](const LowpScanArgs& x, uint32_t nsl, bool minonly) {
+    return fp8 ? launch_scan_fp8(x, metric, nsl, minonly, stream_)
+               : launch_scan_bf16(x, metric, nsl, minonly, stream_);
+  };
+  LowpScanArgs sa{fp8 ? (const uint16_t*)sh8_.X8 : sh_.Xb, dp, w0, w1, (const uint16_t*)s_qb_,
+                  (const QParams*)s_qp_, nq, rps, rps, nrows_pad, NL, (float*)s_ck_,
+                  (uint32_t*)s_cs_, nullptr, 0};
   if (pre) {
     LowpScanArgs pa = sa;
     pa.rows_per_slice = tp * kLowpTile;
     pa.slice_stride = (tiles / ns_pre) * kLowpTile;
     pa.NL = NL_pre;
     prof_begin(SVDB_K_PREPASS);
-    HIPCHK(launch_scan_bf16(pa, metric, ns_pre, true, stream_), "k_scan_bf16 (sample)");
+    HIPCHK(scan(pa, ns_pre, true), "first-pass scan (sample)");
     // thresholds in s_beta_, read by the main scan before the shortlist
     // below overwrites them
     HIPCHK(launch_kth_key((const float*)s_ck_, nq, NL_pre, L, (float*)s_beta_, stream_),
@@ -645,7 +745,7 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
     sa.tau0_stride = 1;
   }
   prof_begin(SVDB_K_SCAN_LOWP);
-  HIPCHK(launch_scan_bf16(sa, metric, ns, false, stream_), "k_scan_bf16");
+  HIPCHK(scan(sa, ns, false), "first-pass scan");
   prof_end(SVDB_K_SCAN_LOWP);
   ShortlistArgs sl{(const float*)s_ck_, (const uint32_t*)s_cs_, nq, NL, L, (uint32_t*)s_ss_,
                    (float*)s_sk_, (float*)s_beta_};
@@ -654,7 +754,7 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   prof_end(SVDB_K_SHORTLIST);
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
   RerankArgs ra{sv, dQ, ldq, (uint32_t)d_, (const QParams*)s_qp_, (const uint32_t*)s_ss_,
-                (const float*)s_sk_, (const float*)s_beta_, sh_.degen, sh_.degen_count, w0, w1,
+                (const float*)s_sk_, (const float*)s_beta_, degen, degen_count, w0, w1,
                 nq, L, (uint32_t)k, ostride, dids, dscores, fail_count, fail_list,
                 (double*)s_err_};
   prof_begin(SVDB_K_RERANK);
@@ -662,7 +762,44 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   prof_end(SVDB_K_RERANK);
   HIPCHK(hipMemcpyAsync(h_fail_, fail_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "copy");
   HIPCHK(hipStreamSynchronize(stream_), "lowp");
-  const uint32_t nf = *h_fail_;
+  const uint32_t nf1 = *h_fail_;
+  uint32_t nf = nf1;
+  // Widening retry: queries not certified at L are shortlisted again at
+  // kMaxShort from the same candidate lists (when they hold at least twice
+  // that many entries) and reranked; only those still uncertified are
+  // rescanned in FP64.
+  const uint32_t Lw = (uint32_t)kMaxShort;
+  if (nf > 0 && L < Lw && (size_t)NL * kLowpLP >= 2 * (size_t)Lw) {
+    if ((rc = scratch(&s_sk2_, &s_sk2_n_, (size_t)nf * Lw * sizeof(float)))) return rc;
+    if ((rc = scratch(&s_ss2_, &s_ss2_n_, (size_t)nf * Lw * sizeof(uint32_t)))) return rc;
+    if ((rc = scratch(&s_beta2_, &s_beta2_n_, (size_t)nf * sizeof(float)))) return rc;
+    if ((rc = scratch(&s_fail2_, &s_fail2_n_, ((size_t)nf + 1) * sizeof(uint32_t)))) return rc;
+    uint32_t* fail2_count = (uint32_t*)s_fail2_;
+    uint32_t* fail2_list = fail2_count + 1;
+    HIPCHK(hipMemsetAsync(fail2_count, 0, sizeof(uint32_t), stream_), "memset");
+    ShortlistArgs sw{(const float*)s_ck_, (const uint32_t*)s_cs_, nf, NL, Lw, (uint32_t*)s_ss2_,
+                     (float*)s_sk2_, (float*)s_beta2_, fail_list};
+    prof_begin(SVDB_K_SHORTLIST);
+    HIPCHK(launch_shortlist(sw, stream_), "k_shortlist (widened)");
+    prof_end(SVDB_K_SHORTLIST);
+    RerankArgs rw = ra;
+    rw.short_slot = (const uint32_t*)s_ss2_;
+    rw.short_key = (const float*)s_sk2_;
+    rw.beta = (const float*)s_beta2_;
+    rw.nq = nf;
+    rw.L = Lw;
+    rw.fail_count = fail2_count;
+    rw.fail_list = fail2_list;
+    rw.qidx = fail_list;
+    prof_begin(SVDB_K_RERANK);
+    HIPCHK(launch_rerank(rw, metric, stream_), "k_rerank (widened)");
+    prof_end(SVDB_K_RERANK);
+    HIPCHK(hipMemcpyAsync(h_fail_, fail2_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+           "copy");
+    HIPCHK(hipStreamSynchronize(stream_), "lowp widened");
+    nf = *h_fail_;
+    fail_list = fail2_list;
+  }
   if (nf > 0) {
     if ((rc = scratch(&s_fq_, &s_fq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
     if ((rc = scratch(&s_fid_, &s_fid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
@@ -687,23 +824,27 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   st_.last_max_rel_err = mx;
   st_.certified += nq - nf;
   st_.fallbacks += nf;
+  st_.widened += nf1 - nf;
   return SVDB_OK;
 }
 
-bool Engine::use_lowp(int precision, size_t D, int* rc) {
+int Engine::use_lowp(int precision, size_t D, int* rc) {
   *rc = SVDB_OK;
-  if (precision == SVDB_PREC_FP64) return false;
-  if (precision == SVDB_PREC_FP8) {
-    set_error(SVDB_EINVAL, "FP8 first pass not available for this store");
-    *rc = SVDB_EINVAL;
-    return false;
-  }
-  const bool ok = lowp_ && D == d_;
-  if (precision == SVDB_PREC_BF16 && !ok) {
+  if (precision == SVDB_PREC_FP64) return 0;
+  const bool bf = lowp_ && D == d_, f8 = lowp8_ && D == d_;
+  if (precision == SVDB_PREC_BF16 && !bf) {
     set_error(SVDB_EINVAL, "BF16 first pass needs the shadow (dimension <= 256) and all columns");
     *rc = SVDB_EINVAL;
+    return 0;
   }
-  return ok;
+  if (precision == SVDB_PREC_FP8 && !f8) {
+    set_error(SVDB_EINVAL, "FP8 first pass needs the FP8 shadow (svdb_enable_shadow) and all columns");
+    *rc = SVDB_EINVAL;
+    return 0;
+  }
+  if (precision == SVDB_PREC_FP8) return 2;
+  if (bf) return 1;
+  return (precision == SVDB_PREC_AUTO && f8) ? 2 : 0;
 }
 
 static int check_search(size_t d, size_t D, int metric, int precision) {
@@ -745,13 +886,14 @@ int Engine::search_device(const double* dQ, size_t nq, size_t ldq, size_t D, siz
   if ((rc = scratch(&s_k_, &s_k_n_, nb * sizeof(int)))) return rc;
   std::vector<int> hk(nb, (int)k);
   HIPCHK(hipMemcpyAsync(s_k_, hk.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
-  const bool lowp = use_lowp(precision, D, &rc) && nphys_ > 0;
+  const int which = nphys_ > 0 ? use_lowp(precision, D, &rc) : 0;
   if (rc) return rc;
+  const bool lowp = which != 0;
   const size_t step = lowp ? kMaxBatchLowp : kMaxBatch;
   for (size_t q0 = 0; q0 < nq; q0 += step) {
     const uint32_t n = (uint32_t)std::min(step, nq - q0);
     rc = lowp ? run_lowp(dQ + q0 * ldq, ldq, n, metric, (int)k, dids + q0 * k, dscores + q0 * k,
-                         (uint32_t)k)
+                         (uint32_t)k, which == 2)
               : run_exact(dQ + q0 * ldq, ldq, n, (uint32_t)D, (const int*)s_k_, metric, (int)k,
                           dids + q0 * k, dscores + q0 * k, (uint32_t)k);
     if (rc) return rc;
@@ -794,8 +936,9 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   int rc;
   for (size_t i = 0; i < nq; i++)
     if ((rc = check_search(d_, D, metric[i], precision))) return rc;
-  const bool lowp = use_lowp(precision, D, &rc) && nphys_ > 0;
+  const int which = nphys_ > 0 ? use_lowp(precision, D, &rc) : 0;
   if (rc) return rc;
+  const bool lowp = which != 0;
   const size_t step = lowp ? kMaxBatchLowp : kMaxBatch;
 
   std::vector<double> hq;
@@ -825,7 +968,7 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
       HIPCHK(hipMemcpyAsync(s_q_, hq.data(), n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
       HIPCHK(hipMemcpyAsync(s_k_, hk.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
       rc = lowp ? run_lowp((const double*)s_q_, D, (uint32_t)n, m, (int)kmax, (uint64_t*)s_ids_,
-                           (double*)s_sc_, (uint32_t)kmax)
+                           (double*)s_sc_, (uint32_t)kmax, which == 2)
                 : run_exact((const double*)s_q_, D, (uint32_t)n, (uint32_t)D, (const int*)s_k_, m,
                             (int)kmax, (uint64_t*)s_ids_, (double*)s_sc_, (uint32_t)kmax);
       if (rc) return rc;

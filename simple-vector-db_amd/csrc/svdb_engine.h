@@ -55,7 +55,7 @@ struct KernelTime {
 };
 
 struct EngineStats {
-  uint64_t searches = 0, queries = 0, certified = 0, fallbacks = 0;
+  uint64_t searches = 0, queries = 0, certified = 0, fallbacks = 0, widened = 0;
   double last_max_rel_err = 0.0;
 };
 
@@ -96,6 +96,7 @@ class Engine {
   int set_profiling(bool on);
   int enable_shadow(int precision);
   bool shadow_on() const { return lowp_; }
+  bool shadow8_on() const { return lowp8_; }
   void kernel_times(KernelTime* out);
   EngineStats stats();
 
@@ -111,9 +112,11 @@ class Engine {
   int run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk, int metric,
                 int kmax, uint64_t* dids, double* dscores, uint32_t ostride);
   int run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k, uint64_t* dids,
-               double* dscores, uint32_t ostride);
-  bool use_lowp(int precision, size_t D, int* rc);
+               double* dscores, uint32_t ostride, bool fp8);
+  // 0: FP64 scan, 1: BF16 first pass, 2: FP8 first pass
+  int use_lowp(int precision, size_t D, int* rc);
   int alloc_shadow(size_t cap);
+  int alloc_shadow8(size_t cap);
   int shadow_rows(uint32_t first, uint32_t n, const uint32_t* dslots);
   uint32_t select_slot(size_t logical) const;
   void prof_begin(int which);
@@ -145,6 +148,8 @@ class Engine {
   static constexpr uint32_t kPrepassSlices = 256;  // 64 on stores of under 16*256 tiles
   static constexpr uint32_t kPrepassMaxTiles = 8;
   Shadow sh_;
+  bool lowp8_ = false;  // FP8 shadow present (built on request)
+  Shadow8 sh8_;
 
   size_t dev_bytes_ = 0;
   size_t budget_ = SIZE_MAX;
@@ -173,6 +178,10 @@ class Engine {
   void* s_fid_ = nullptr;    size_t s_fid_n_ = 0;
   void* s_fsc_ = nullptr;    size_t s_fsc_n_ = 0;
   void* s_fk_ = nullptr;     size_t s_fk_n_ = 0;
+  void* s_sk2_ = nullptr;    size_t s_sk2_n_ = 0;   // widening retry: shortlist keys
+  void* s_ss2_ = nullptr;    size_t s_ss2_n_ = 0;   //   slots
+  void* s_beta2_ = nullptr;  size_t s_beta2_n_ = 0; //   bounds
+  void* s_fail2_ = nullptr;  size_t s_fail2_n_ = 0; //   queries still uncertified
   uint32_t* h_fail_ = nullptr;  // pinned: fail count
   void* h_stage_ = nullptr;     // pinned: row upload staging
   size_t h_stage_n_ = 0;

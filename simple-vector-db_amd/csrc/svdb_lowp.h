@@ -34,7 +34,7 @@ constexpr int kLowpTile = 128;                     // rows per LDS tile
 constexpr int kLowpLP = 8;                         // entries per per-lane list
 constexpr int kLowpLanesPerQuery = 4;              // lists per query per slice
 constexpr int kDegenCap = 64;                      // degenerate rows kept in the list
-constexpr int kMaxShort = 256;                     // widest shortlist
+constexpr int kMaxShort = 1024;                    // widest shortlist (the widening retry)
 
 struct Shadow {
   uint16_t* Xb = nullptr;   // [cap][dp]
@@ -44,6 +44,28 @@ struct Shadow {
   float* inv = nullptr;
   float* pen = nullptr;
   uint32_t* rho_bits = nullptr;   // fp32 bits of ρ (running max)
+  uint32_t* degen_count = nullptr;
+  uint32_t* degen = nullptr;      // [kDegenCap]
+};
+
+// FP8 (OCP e4m3fn) shadow with per-row power-of-two scales:
+//   X8      e4m3 row of x / sx, zero-padded to dp = round_up(d, 256)
+//   sx      fp32 2^e with max|x| / sx in (224, 448]
+//   nx, xn  as in Shadow
+//   inv     fp32 sx / ‖x‖ (cosine ordering key, sx folded in)
+//   pen     as in Shadow
+// x̂ = sx·e4m3(x/sx) exactly, so every FP8 ordering key is an exact
+// power-of-two rescaling of the BF16-form key and the same bound applies
+// with this shadow's ρ.
+struct Shadow8 {
+  uint8_t* X8 = nullptr;    // [cap][dp]
+  size_t dp = 0;
+  float* sx = nullptr;
+  float* nx = nullptr;
+  float* xn = nullptr;
+  float* inv = nullptr;
+  float* pen = nullptr;
+  uint32_t* rho_bits = nullptr;
   uint32_t* degen_count = nullptr;
   uint32_t* degen = nullptr;      // [kDegenCap]
 };
@@ -88,6 +110,7 @@ struct ShortlistArgs {
   uint32_t* short_slot;     // [nq][L]  (UINT32_MAX when fewer)
   float* short_key;         // [nq][L]
   float* beta;              // [nq] bound on the ordering key of rows left out
+  const uint32_t* qidx;     // optional: row i of the outputs is query qidx[i]
 };
 
 struct RerankArgs {
@@ -110,6 +133,7 @@ struct RerankArgs {
   uint32_t* fail_count;     // queries not certified
   uint32_t* fail_list;      // their indices
   double* max_rel_err;      // [nq] first-pass score error of the returned scores
+  const uint32_t* qidx;     // optional: shortlist row i is query qidx[i]
 };
 
 // Queries per block of the scan for a padded row length dp.
@@ -130,6 +154,25 @@ hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices,
 // fewer than L are finite).  One wavefront per query.
 hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L, float* thr,
                           hipStream_t s);
+// ---- FP8 first pass (svdb_fp8.hip) ----
+// FP8 rows are padded to a multiple of 256 bytes; supported padded lengths.
+bool fp8_dp_supported(size_t dp);
+int fp8_queries_per_block(size_t dp);
+hipError_t launch_shadow_init8(const Shadow8& sh, uint32_t first, uint32_t n, hipStream_t s);
+hipError_t launch_quantize_rows8(const Shadow8& sh, const double* X, size_t ldx, const double* nx2,
+                                 const double* xn, const uint32_t* slots, uint32_t first,
+                                 uint32_t n, uint32_t d, hipStream_t s);
+hipError_t launch_kill_rows8(const Shadow8& sh, const uint32_t* slots, uint32_t n, hipStream_t s);
+// Q8: e4m3 queries [nq_pad][dp]; QParams as for BF16 with P / scale carrying
+// the query's power-of-two scale (see svdb_fp8.hip).
+hipError_t launch_query_prep8(const double* Q, size_t ldq, uint32_t nq, uint32_t nq_pad,
+                              uint32_t d, size_t dp, int metric, const uint32_t* rho_bits,
+                              uint8_t* Q8, QParams* qp, hipStream_t s);
+// a.Xb / a.Qb point at e4m3 bytes; a.w0 / a.w1: L2 (nx, sx), dot (xn, sx),
+// cosine (inv, pen).
+hipError_t launch_scan_fp8(const LowpScanArgs& a, int metric, uint32_t nslices, bool minonly,
+                           hipStream_t s);
+
 hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s);
 hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s);
 hipError_t launch_gather_rows(const double* src, size_t lds, const uint32_t* idx, uint32_t n,

@@ -293,13 +293,15 @@ hipError_t launch_query_prep(const double* Q, size_t ldq, uint32_t nq, uint32_t 
 // beta = min(8th entry of every list, L-th merged key): no row outside the
 // shortlist has a smaller ordering key.
 
+template <int KPL>
 __global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
   const int lane = threadIdx.x & 63;
-  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (q >= a.nq) return;
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= a.nq) return;
+  const uint32_t q = a.qidx ? a.qidx[i] : i;  // candidate lists of query q -> output row i
   const uint32_t n = a.NL * kLowpLP;
   const size_t base = (size_t)q * n;
-  WL<kMaxShort / 64> L;
+  WL<KPL> L;
   L.init();
   const int kk = (int)a.L;
   uint64_t tk = ~0ull;
@@ -344,21 +346,25 @@ __global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
     oL = __uint_as_float((u >> 31) ? (u & 0x7FFFFFFFu) : ~u);
   }
 #pragma unroll
-  for (int p = 0; p < kMaxShort / 64; p++) {
+  for (int p = 0; p < KPL; p++) {
     const int e = p * 64 + lane;
     if (e < kk) {
       const uint32_t u = (uint32_t)L.k[p];
-      a.short_slot[(size_t)q * a.L + e] = L.s[p];
-      a.short_key[(size_t)q * a.L + e] =
+      a.short_slot[(size_t)i * a.L + e] = L.s[p];
+      a.short_key[(size_t)i * a.L + e] =
           L.s[p] == ~0u ? INFINITY : __uint_as_float((u >> 31) ? (u & 0x7FFFFFFFu) : ~u);
     }
   }
-  if (lane == 0) a.beta[q] = fminf(lane_min, oL);
+  if (lane == 0) a.beta[i] = fminf(lane_min, oL);
 }
 
 hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s) {
   if (a.nq == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shortlist, dim3((a.nq + 3) / 4), dim3(256), 0, s, a);
+  if (a.L == 0 || a.L > (uint32_t)kMaxShort) return hipErrorInvalidValue;
+  const dim3 g((a.nq + 3) / 4), b(256);
+  if (a.L <= 64) hipLaunchKernelGGL(k_shortlist<1>, g, b, 0, s, a);
+  else if (a.L <= 256) hipLaunchKernelGGL(k_shortlist<4>, g, b, 0, s, a);
+  else hipLaunchKernelGGL(k_shortlist<kMaxShort / 64>, g, b, 0, s, a);
   return hipGetLastError();
 }
 
@@ -418,8 +424,10 @@ hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L,
 template <int METRIC>
 __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   const int lane = threadIdx.x & 63;
-  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (q >= a.nq) return;
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= a.nq) return;
+  // shortlist row i belongs to query q (its parameters, FP64 row, outputs)
+  const uint32_t q = a.qidx ? a.qidx[i] : i;
   const QParams qp = a.qp[q];
   const double* __restrict__ qv = a.Q + (size_t)q * a.ldq;
   const uint32_t ndeg = min(*a.degen_count, (uint32_t)kDegenCap);
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   for (uint32_t j0 = 0; j0 < ncand; j0 += 64) {
     const uint32_t j = j0 + lane;
     uint32_t slot = ~0u;
-    if (j < a.L) slot = a.short_slot[(size_t)q * a.L + j];
+    if (j < a.L) slot = a.short_slot[(size_t)i * a.L + j];
     else if (j < ncand) slot = a.degen[j - a.L];
     bool valid = slot != ~0u && slot < a.st.nrows &&
                  ((a.st.alive[slot >> 6] >> (slot & 63)) & 1ull);
@@ -457,7 +465,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
     // a degenerate row that is also shortlisted must not enter twice
     if (valid && j >= a.L) {
       for (uint32_t t = 0; t < a.L; t++)
-        if (a.short_slot[(size_t)q * a.L + t] == slot) valid = false;
+        if (a.short_slot[(size_t)i * a.L + t] == slot) valid = false;
     }
     uint64_t m = __ballot(valid && lt2(K, slot, tk, ts));
     while (m) {
@@ -472,7 +480,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   // more degenerate rows than the list holds: nothing can be certified
   bool cert = false;
   if (!qp.flags && *a.degen_count <= (uint32_t)kDegenCap) {
-    const float bo = a.beta[q];
+    const float bo = a.beta[i];
     double bR;
     if (bo == INFINITY) {
       bR = INFINITY;
@@ -510,15 +518,16 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
         // first-pass estimate of this score, for the error report
         float O = INFINITY;
         for (uint32_t t = 0; t < a.L; t++)
-          if (a.short_slot[(size_t)q * a.L + t] == slot) O = a.short_key[(size_t)q * a.L + t];
+          if (a.short_slot[(size_t)i * a.L + t] == slot) O = a.short_key[(size_t)i * a.L + t];
         if (O != INFINITY && sc == sc && sc != 0.0) {
           const double xn = a.st.xn[slot];
           double est;
           if (METRIC == kL2) {
-            const double sp = (double)O - (double)qp.P * (double)a.w0[slot];
+            // (FP8 keys are the BF16-form key / sq; scale = sq there, 1 for BF16)
+            const double sp = ((double)O - (double)qp.P * (double)a.w0[slot]) * qp.scale;
             est = sqrt(fmax(qp.nq2 + xn * xn + sp, 0.0));
           } else if (METRIC == kDot) {
-            est = -((double)O - (double)qp.P * (double)a.w0[slot]);
+            est = -((double)O - (double)qp.P * (double)a.w0[slot]) * qp.scale;
           } else {
             est = -(double)O / qp.qn;
           }

@@ -57,7 +57,8 @@ class Stats(C.Structure):
     _fields_ = [("rows", C.c_size_t), ("dim", C.c_size_t), ("phys_rows", C.c_size_t),
                 ("device_bytes", C.c_size_t), ("budget_bytes", C.c_size_t),
                 ("searches", C.c_uint64), ("queries", C.c_uint64), ("certified", C.c_uint64),
-                ("fallbacks", C.c_uint64), ("last_max_rel_err", C.c_double)]
+                ("fallbacks", C.c_uint64), ("last_max_rel_err", C.c_double),
+                ("widened", C.c_uint64)]
 
 
 class KernelTime(C.Structure):

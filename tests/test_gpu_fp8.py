@@ -1,0 +1,154 @@
+"""GPU parity of the FP8 (OCP e4m3) first pass + FP64 rerank, through the C ABI.
+
+Bar: returned ids and FP64 scores bit-identical to the CPU FP64 oracle on
+wide rows (the wide-vector workload's shapes, d = 768 and 1536, at sizes the
+oracle checks in seconds) and on the other supported padded widths.  The
+certification keeps the ids exact whatever the first pass does, so the tests
+also check that the first pass is really FP8: its score error sits at e4m3
+level (well above BF16's, well below 1), and that most top-10 queries are
+certified without the FP64 rescan.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import svdb
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    a = np.array(a, dtype=np.float64)
+    a[np.isnan(a)] = np.nan
+    return a.view(np.uint64)
+
+
+def assert_same(ids, sc, rid, rsc, what=""):
+    bad = np.argwhere(ids != rid)
+    assert bad.size == 0, f"{what}: ids differ at {bad[:5].tolist()}"
+    assert np.array_equal(bits(sc), bits(rsc)), f"{what}: scores differ"
+
+
+def store(X, fp8=True):
+    db = svdb.Database(kd_dimension=3)
+    db.bulk_insert(X)
+    if fp8:
+        db.enable_shadow("fp8")
+    return db
+
+
+@pytest.fixture(scope="module")
+def wide768():
+    # cluster mixture with LogNormal(0, 1) norms (dot ranking is the hard case)
+    X = workloads.mixture(60_000, 768, seed=51, n_centers=64, lognorm=1.0)
+    Q = workloads.mixture(96, 768, seed=52, n_centers=64, lognorm=1.0)
+    say("fp8 store: 60000x768 mixture, LogNormal(0,1) norms")
+    db = store(X)
+    yield X, Q, db
+    db.close()
+
+
+@pytest.mark.parametrize("metric", ["l2", "cos", "dot"])
+@pytest.mark.parametrize("k", [1, 10, 100])
+def test_fp8_exact_768(wide768, metric, k):
+    X, Q, db = wide768
+    say(f"fp8 768 {metric} k={k}")
+    s0 = db.stats()
+    ids, sc = db.search(Q, k, metric, precision="fp8")
+    s1 = db.stats()
+    rid, rsc = O.topk(X, Q, k, metric)
+    assert_same(ids, sc, rid, rsc, f"fp8 768 {metric} k={k}")
+    cert = s1["certified"] - s0["certified"]
+    fb = s1["fallbacks"] - s0["fallbacks"]
+    wid = s1["widened"] - s0["widened"]
+    say(f"  certified {cert} ({wid} by the 1024-wide retry), fp64 rescans {fb}, "
+        f"first-pass err {s1['last_max_rel_err']:.3g}")
+    assert cert + fb == Q.shape[0]
+    if k <= 10:
+        assert cert >= 0.5 * Q.shape[0]
+    if cert:
+        # e4m3 carries 3 mantissa bits: errors of a few percent of a score
+        assert 1e-4 < s1["last_max_rel_err"] < 0.5
+
+
+def test_fp8_1536_and_auto():
+    X = workloads.mixture(20_000, 1536, seed=53, n_centers=32, lognorm=1.0)
+    Q = workloads.mixture(48, 1536, seed=54, n_centers=32, lognorm=1.0)
+    db = store(X)
+    for metric in ("l2", "cos", "dot"):
+        say(f"fp8 1536 {metric}")
+        ids, sc = db.search(Q, 10, metric, precision="fp8")
+        rid, rsc = O.topk(X, Q, 10, metric)
+        assert_same(ids, sc, rid, rsc, f"fp8 1536 {metric}")
+    # no BF16 shadow at d > 256: "auto" takes the FP8 first pass
+    s0 = db.stats()
+    ids, sc = db.search(Q, 10, "dot", precision="auto")
+    assert db.stats()["certified"] + db.stats()["fallbacks"] > s0["certified"] + s0["fallbacks"]
+    rid, rsc = O.topk(X, Q, 10, "dot")
+    assert_same(ids, sc, rid, rsc, "fp8 1536 auto")
+    db.close()
+
+
+@pytest.mark.parametrize("d", [100, 256, 300, 1000])
+def test_fp8_other_widths(d):
+    # padded widths 256, 256, 512, 1024 (K steps 2, 2, 4, 8)
+    rng = np.random.default_rng(d)
+    X = rng.standard_normal((12_000, d))
+    Q = rng.standard_normal((40, d))
+    db = store(X)
+    for metric in ("l2", "dot"):
+        say(f"fp8 d={d} {metric}")
+        ids, sc = db.search(Q, 5, metric, precision="fp8")
+        rid, rsc = O.topk(X, Q, 5, metric)
+        assert_same(ids, sc, rid, rsc, f"fp8 d={d} {metric}")
+    db.close()
+
+
+def test_fp8_mutations_and_degenerate_rows():
+    rng = np.random.default_rng(9)
+    d = 768
+    X = workloads.mixture(8_000, d, seed=55, n_centers=16, lognorm=1.0)
+    db = store(X)
+    rows = [r for r in X]
+    # updates, deletes, inserts, and rows the first pass cannot scale
+    for i in (5, 77, 4000):
+        v = rng.standard_normal(d)
+        db.update(i, v)
+        rows[i] = v
+    for i in (3, 3, 1000):
+        db.delete(i)
+        del rows[i]
+    special = [np.zeros(d), np.full(d, 1e30), np.full(d, 1e-30), rng.standard_normal(d)]
+    special[3][7] = np.nan
+    for v in special + [rng.standard_normal(d) for _ in range(5)]:
+        db.insert(v)
+        rows.append(v)
+    Xm = np.stack(rows)
+    assert len(db) == Xm.shape[0]
+    Q = np.vstack([Xm[[10, 200, 7000]] + 1e-3, rng.standard_normal((5, d))])
+    for metric in ("l2", "cos", "dot"):
+        say(f"fp8 mutations {metric}")
+        ids, sc = db.search(Q, 10, metric, precision="fp8")
+        rid, rsc = O.topk(Xm, Q, 10, metric)
+        assert_same(ids, sc, rid, rsc, f"fp8 mutations {metric}")
+    db.close()
+
+
+def test_fp8_enable_errors():
+    db = svdb.Database(kd_dimension=3)
+    with pytest.raises(svdb.SvdbError):
+        db.enable_shadow("fp8")  # no rows yet
+    db.bulk_insert(np.random.default_rng(1).standard_normal((100, 2000)))
+    with pytest.raises(svdb.SvdbError):
+        db.enable_shadow("fp8")  # wider than 1536
+    with pytest.raises(svdb.SvdbError):
+        db.search(np.zeros((1, 2000)), 1, "l2", precision="fp8")
+    db.close()
