@@ -161,6 +161,96 @@ def serving_sweep(db, pool, store_rows, seconds, loads=(0.3, 0.5, 0.7, 0.9)):
     return sat, levels
 
 
+def wide_bench(args):
+    """Wide vectors with an FP8 first pass: 5,000,000 x 768 and 2,000,000 x 1536
+    cluster mixtures (1,024 clusters, sigma 0.05, LogNormal(0, 1) norms), top-100
+    for batches of 4,096 device-resident queries, FP8 e4m3 first pass with a
+    128-wide shortlist (widened to 1,024 for queries it cannot certify) and
+    FP64 rerank.  Ids are checked against the plain FP64 GPU scan on a subset
+    of each batch and against the CPU oracle on a small sample."""
+    import svdb
+    import workloads
+    out = []
+    shapes = [(int(r), int(d)) for r, d in (x.split("x") for x in args.wide.split(","))]
+    for rows, d in shapes:
+        t = time.time()
+        X = workloads.mixture(rows, d, 61, lognorm=1.0)
+        Q = workloads.mixture(args.wide_batch, d, 62, lognorm=1.0)
+        log(f"wide {rows}x{d}: generated in {time.time() - t:.1f}s")
+        db = svdb.Database(kd_dimension=3)
+        t = time.time()
+        db.bulk_insert(X)
+        db.enable_shadow("fp8")
+        db.set_shortlist(128)
+        log(f"wide {rows}x{d}: loaded + FP8 shadow in {time.time() - t:.1f}s")
+        B, k = Q.shape[0], 100
+        dQ = svdb.DeviceArray.from_host(Q)
+        dids = svdb.DeviceArray((B, k), np.uint64)
+        dsc = svdb.DeviceArray((B, k), np.float64)
+        res = {"rows": rows, "dim": d, "batch": B, "k": k, "shortlist": 128}
+        for metric in ("dot", "l2", "cos"):
+            db.search_device(dQ.ptr, B, d, k, metric, "fp8", dids.ptr, dsc.ptr)  # warm
+            svdb.device_sync()
+            db.set_profiling(True)
+            s0 = db.stats()
+            t0 = time.perf_counter()
+            for _ in range(args.wide_steps):
+                db.search_device(dQ.ptr, B, d, k, metric, "fp8", dids.ptr, dsc.ptr)
+            svdb.device_sync()
+            el = (time.perf_counter() - t0) / args.wide_steps
+            s1 = db.stats()
+            kt = db.kernel_times()
+            ids, sc = dids.to_host(), dsc.to_host()
+            n = s1["queries"] - s0["queries"]
+            cert = s1["certified"] - s0["certified"]
+            wid = s1["widened"] - s0["widened"]
+            fb = s1["fallbacks"] - s0["fallbacks"]
+            scan_ms = kt[svdb.K_SCAN_LOWP][1] / max(1, kt[svdb.K_SCAN_LOWP][0])
+            tflops = 2.0 * rows * d * B / (scan_ms * 1e-3) / 1e12
+            # FP64 GPU scan of the first queries: same ids and scores
+            nf = min(args.wide_fp64, B)
+            t1 = time.perf_counter()
+            db.search_device(dQ.ptr, nf, d, k, metric, "fp64", dids.ptr, dsc.ptr)
+            svdb.device_sync()
+            fp64_s = time.perf_counter() - t1
+            fid, fsc = dids.to_host()[:nf], dsc.to_host()[:nf]
+            same = bool(np.array_equal(fid, ids[:nf]) and
+                        np.array_equal(fsc.view(np.uint64), sc[:nf].view(np.uint64)))
+            r = {"qps": round(B / el, 1), "ms_per_batch": round(el * 1e3, 2),
+                 "certified_at_128": cert - wid, "certified_at_1024": wid, "fallback_fp64": fb,
+                 "queries": n,
+                 "scan_fp8_ms": round(scan_ms, 3), "scan_fp8_tflops": round(tflops, 1),
+                 "scan_fp8_frac_of_5pf": round(tflops / 5000.0, 4),
+                 "stage_ms": {nm: round(kt[i][1] / max(1, args.wide_steps), 3) for i, nm in
+                              ((svdb.K_SCAN_LOWP, "scan_fp8"), (svdb.K_PREPASS, "prepass"),
+                               (svdb.K_SHORTLIST, "shortlists"), (svdb.K_RERANK, "reranks"),
+                               (svdb.K_SCAN_FP64, "fp64_rescan"))},
+                 "fp64_gpu_scan_qps": round(nf / fp64_s, 1),
+                 "ids_scores_equal_fp64_gpu_scan": same, "fp64_checked": nf,
+                 "max_first_pass_rel_err": s1["last_max_rel_err"]}
+            if not args.no_cpu and args.wide_cpu > 0:
+                sys.path.insert(0, os.path.join(ROOT, "oracle"))
+                import oracle
+                m = args.wide_cpu
+                t2 = time.perf_counter()
+                rid, rsc = oracle.topk(X, Q[:m], k, metric,
+                                       threads=min(16, len(os.sched_getaffinity(0))))
+                r["cpu_oracle_qps"] = round(m / (time.perf_counter() - t2), 3)
+                r["ids_scores_equal_cpu_oracle"] = bool(
+                    np.array_equal(rid, ids[:m]) and
+                    np.array_equal(rsc.view(np.uint64), sc[:m].view(np.uint64)))
+            log(f"wide {rows}x{d} {metric}: {r}")
+            res[metric] = r
+        out.append(res)
+        for a in (dQ, dids, dsc):
+            a.free()
+        db.close()
+        del X
+    print(json.dumps({"metric": "wide-vector FP8 first pass, top-100 queries/s", "unit": "queries/s",
+                      "data": "synthetic (host numpy PCG64, cluster mixture, LogNormal(0,1) norms)",
+                      "results": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +271,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (twice) and exit: "
                          "the known-traffic reference for the HBM byte counters")
+    ap.add_argument("--wide", default="",
+                    help="run only the wide-vector FP8 workload on these shapes, e.g. "
+                         "5000000x768,2000000x1536")
+    ap.add_argument("--wide-batch", type=int, default=4096)
+    ap.add_argument("--wide-steps", type=int, default=3)
+    ap.add_argument("--wide-fp64", type=int, default=64)
+    ap.add_argument("--wide-cpu", type=int, default=4)
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -197,6 +294,10 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.wide:
+        wide_bench(args)
+        return
 
     import svdb
     import workloads

@@ -656,11 +656,12 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   const size_t qbytes = fp8 ? 1 : sizeof(uint16_t);  // bytes per query element
   const uint32_t qpb = (uint32_t)(fp8 ? fp8_queries_per_block(dp) : lowp_queries_per_block(dp));
   const uint32_t nq_pad = (uint32_t)round_up(nq, qpb);
-  // Shortlist: the configured width (64 by default) for small k; 256 once
-  // k passes 32, where the k-th and L-th neighbours need more room than the
-  // first-pass error bound leaves in a 2k-wide list.
+  // Shortlist: the configured width (64 by default) for small k; for BF16
+  // 256 once k passes 32, where the k-th and L-th neighbours need more room
+  // than the first-pass error bound leaves in a 2k-wide list.  FP8 starts at
+  // the configured width (or k) and relies on the 1024-wide retry below.
   const uint32_t L = (uint32_t)std::min<size_t>(
-      kMaxShort, std::max<size_t>({shortlist_, (size_t)k, k > 32 ? (size_t)kMaxShort : 0}));
+      kMaxShort, std::max<size_t>({shortlist_, (size_t)k, (!fp8 && k > 32) ? (size_t)256 : 0}));
   const uint32_t nrows_pad = (uint32_t)round_up(nphys_, kLowpTile);
   // Row slices: at least 8 (each query then has >= 32 candidate lists, so
   // the lists' own bound sits well past the shortlist's), more when there
@@ -671,9 +672,10 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   // Past k = 32 the lists' own bound (the smallest 8th entry) must stay
   // well beyond the k-th neighbour: at least 256 lists per query.
   if (k > 32) ns = std::max<uint32_t>(ns, 64);
-  // FP8: the widening retry takes up to kMaxShort candidates from these
-  // lists, so keep at least 512 of them per query.
-  if (fp8) ns = std::max<uint32_t>(ns, 128);
+  // FP8: the widening retries take up to kMaxShort candidates from these
+  // lists; 2048 lists per query keep their own bound (the smallest 8th
+  // entry) clear of the 4096th candidate.
+  if (fp8) ns = std::max<uint32_t>(ns, 512);
   ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, tiles / 4)));
   const uint32_t rps = (uint32_t)round_up((tiles + ns - 1) / ns, 1) * kLowpTile;
   ns = (nrows_pad + rps - 1) / rps;
@@ -764,19 +766,20 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   HIPCHK(hipStreamSynchronize(stream_), "lowp");
   const uint32_t nf1 = *h_fail_;
   uint32_t nf = nf1;
-  // Widening retry: queries not certified at L are shortlisted again at
-  // kMaxShort from the same candidate lists (when they hold at least twice
-  // that many entries) and reranked; only those still uncertified are
+  // Widening retries: queries not certified at L are shortlisted again at
+  // 1024, then 4096, from the same candidate lists (when they hold at least
+  // twice that many entries) and reranked; only those still uncertified are
   // rescanned in FP64.
-  const uint32_t Lw = (uint32_t)kMaxShort;
-  if (nf > 0 && L < Lw && (size_t)NL * kLowpLP >= 2 * (size_t)Lw) {
+  for (uint32_t Lw : {1024u, (uint32_t)kMaxShort}) {
+    if (nf == 0 || Lw <= L || (size_t)NL * kLowpLP < 2 * (size_t)Lw) continue;
     if ((rc = scratch(&s_sk2_, &s_sk2_n_, (size_t)nf * Lw * sizeof(float)))) return rc;
     if ((rc = scratch(&s_ss2_, &s_ss2_n_, (size_t)nf * Lw * sizeof(uint32_t)))) return rc;
     if ((rc = scratch(&s_beta2_, &s_beta2_n_, (size_t)nf * sizeof(float)))) return rc;
-    if ((rc = scratch(&s_fail2_, &s_fail2_n_, ((size_t)nf + 1) * sizeof(uint32_t)))) return rc;
-    uint32_t* fail2_count = (uint32_t*)s_fail2_;
-    uint32_t* fail2_list = fail2_count + 1;
-    HIPCHK(hipMemsetAsync(fail2_count, 0, sizeof(uint32_t), stream_), "memset");
+    if ((rc = scratch(&s_fail2_, &s_fail2_n_, 2 * ((size_t)nf + 1) * sizeof(uint32_t)))) return rc;
+    // two fail lists alternate: this stage reads `fail_list`, writes the other
+    uint32_t* out_count = (uint32_t*)s_fail2_ + (fail_list == (uint32_t*)s_fail2_ + 1 ? nf + 1 : 0);
+    uint32_t* out_list = out_count + 1;
+    HIPCHK(hipMemsetAsync(out_count, 0, sizeof(uint32_t), stream_), "memset");
     ShortlistArgs sw{(const float*)s_ck_, (const uint32_t*)s_cs_, nf, NL, Lw, (uint32_t*)s_ss2_,
                      (float*)s_sk2_, (float*)s_beta2_, fail_list};
     prof_begin(SVDB_K_SHORTLIST);
@@ -788,17 +791,17 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
     rw.beta = (const float*)s_beta2_;
     rw.nq = nf;
     rw.L = Lw;
-    rw.fail_count = fail2_count;
-    rw.fail_list = fail2_list;
+    rw.fail_count = out_count;
+    rw.fail_list = out_list;
     rw.qidx = fail_list;
     prof_begin(SVDB_K_RERANK);
     HIPCHK(launch_rerank(rw, metric, stream_), "k_rerank (widened)");
     prof_end(SVDB_K_RERANK);
-    HIPCHK(hipMemcpyAsync(h_fail_, fail2_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+    HIPCHK(hipMemcpyAsync(h_fail_, out_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
            "copy");
     HIPCHK(hipStreamSynchronize(stream_), "lowp widened");
     nf = *h_fail_;
-    fail_list = fail2_list;
+    fail_list = out_list;
   }
   if (nf > 0) {
     if ((rc = scratch(&s_fq_, &s_fq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;

@@ -34,7 +34,7 @@ constexpr int kLowpTile = 128;                     // rows per LDS tile
 constexpr int kLowpLP = 8;                         // entries per per-lane list
 constexpr int kLowpLanesPerQuery = 4;              // lists per query per slice
 constexpr int kDegenCap = 64;                      // degenerate rows kept in the list
-constexpr int kMaxShort = 1024;                    // widest shortlist (the widening retry)
+constexpr int kMaxShort = 4096;                    // widest shortlist (the widening retries)
 
 struct Shadow {
   uint16_t* Xb = nullptr;   // [cap][dp]

@@ -358,13 +358,147 @@ __global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
   if (lane == 0) a.beta[i] = fminf(lane_min, oL);
 }
 
+// Wide shortlists (L > 256): one 256-thread block per query.  The query's
+// n <= 256*kSelPT candidate keys sit in registers as order-preserving
+// uint32 (empty entries: all ones); four 8-bit histogram passes in LDS find
+// the L-th smallest key K*; every entry below K* and enough entries equal to
+// it are written out, unordered (the rerank does not need an order).  A row
+// left out has a key >= K* or >= its list's 8th entry, so
+// beta = min(8th entry of every list, K*) as in the sorted form.
+constexpr int kSelPT = 64;
+
+__global__ __launch_bounds__(256) void k_shortlist_sel(ShortlistArgs a) {
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t s_prefix, s_mask, s_rank, s_nvalid, s_lt, s_eq;
+  __shared__ float s_min[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t i = blockIdx.x;
+  const uint32_t q = a.qidx ? a.qidx[i] : i;
+  const uint32_t n = a.NL * kLowpLP;
+  const size_t base = (size_t)q * n;
+  uint32_t u[kSelPT];
+  float lmin = INFINITY;
+  uint32_t nv = 0;
+#pragma unroll
+  for (int t = 0; t < kSelPT; t++) {
+    const uint32_t c = t * 256 + tid;
+    u[t] = 0xFFFFFFFFu;
+    if (c < n) {
+      const float key = a.cand_key[base + c];
+      if ((c % kLowpLP) == kLowpLP - 1) lmin = fminf(lmin, key);
+      if (a.cand_slot[base + c] != ~0u) {
+        u[t] = rank_bits32(key);
+        nv++;
+      }
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    lmin = fminf(lmin, __shfl_xor(lmin, off));
+    nv += __shfl_xor(nv, off);
+  }
+  if (tid == 0) {
+    s_prefix = 0;
+    s_mask = 0;
+    s_rank = a.L;
+    s_nvalid = 0;
+    s_lt = 0;
+    s_eq = 0;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s_min[wave] = lmin;
+    atomicAdd(&s_nvalid, nv);
+  }
+  __syncthreads();
+  const uint32_t nvalid = s_nvalid;
+  lmin = fminf(fminf(s_min[0], s_min[1]), fminf(s_min[2], s_min[3]));
+  uint32_t kstar = 0xFFFFFFFFu;  // fewer than L valid entries: take them all
+  if (nvalid >= a.L) {
+    for (int pass = 0; pass < 4; pass++) {
+      const int shift = 24 - 8 * pass;
+      hist[tid] = 0;
+      __syncthreads();
+      const uint32_t prefix = s_prefix, mask = s_mask;
+#pragma unroll
+      for (int t = 0; t < kSelPT; t++)
+        if (u[t] != 0xFFFFFFFFu && (u[t] & mask) == prefix)
+          atomicAdd(&hist[(u[t] >> shift) & 255u], 1u);
+      __syncthreads();
+      if (wave == 0) {
+        // lane l holds bins 4l..4l+3; an inclusive scan over lanes finds
+        // the bin where the running count reaches the target rank
+        const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                       h3 = hist[4 * lane + 3];
+        const uint32_t sum = h0 + h1 + h2 + h3;
+        uint32_t inc = sum;
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t v = __shfl_up(inc, off);
+          if (lane >= off) inc += v;
+        }
+        const uint32_t exc = inc - sum, rank = s_rank;
+        if (exc < rank && rank <= inc) {
+          uint32_t cum = exc, bin = 4 * lane;
+          const uint32_t hs[4] = {h0, h1, h2, h3};
+          for (int b = 0; b < 4; b++) {
+            if (cum + hs[b] >= rank) {
+              bin = 4 * lane + b;
+              break;
+            }
+            cum += hs[b];
+          }
+          s_rank = rank - cum;
+          s_prefix = prefix | (bin << shift);
+          s_mask = mask | (255u << shift);
+        }
+      }
+      __syncthreads();
+    }
+    kstar = s_prefix;
+  }
+  // entries below K*: all of them; equal to K*: the first s_rank that come
+  const uint32_t need_eq = nvalid >= a.L ? s_rank : 0u;
+  const uint32_t n_lt = nvalid >= a.L ? a.L - need_eq : 0u;
+#pragma unroll
+  for (int t = 0; t < kSelPT; t++) {
+    if (u[t] == 0xFFFFFFFFu) continue;
+    uint32_t pos = ~0u;
+    if (nvalid < a.L || u[t] < kstar) {
+      pos = atomicAdd(&s_lt, 1u);
+    } else if (u[t] == kstar) {
+      const uint32_t e = atomicAdd(&s_eq, 1u);
+      if (e < need_eq) pos = n_lt + e;
+    }
+    if (pos != ~0u) {
+      const uint32_t c = t * 256 + tid;
+      a.short_slot[(size_t)i * a.L + pos] = a.cand_slot[base + c];
+      a.short_key[(size_t)i * a.L + pos] = a.cand_key[base + c];
+    }
+  }
+  __syncthreads();
+  // unused tail (fewer than L valid entries)
+  for (uint32_t e = (nvalid < a.L ? nvalid : a.L) + tid; e < a.L; e += 256) {
+    a.short_slot[(size_t)i * a.L + e] = ~0u;
+    a.short_key[(size_t)i * a.L + e] = INFINITY;
+  }
+  if (tid == 0) {
+    const float oL =
+        nvalid >= a.L ? __uint_as_float((kstar >> 31) ? (kstar & 0x7FFFFFFFu) : ~kstar) : INFINITY;
+    a.beta[i] = fminf(lmin, oL);
+  }
+}
+
 hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s) {
   if (a.nq == 0) return hipSuccess;
   if (a.L == 0 || a.L > (uint32_t)kMaxShort) return hipErrorInvalidValue;
   const dim3 g((a.nq + 3) / 4), b(256);
-  if (a.L <= 64) hipLaunchKernelGGL(k_shortlist<1>, g, b, 0, s, a);
-  else if (a.L <= 256) hipLaunchKernelGGL(k_shortlist<4>, g, b, 0, s, a);
-  else hipLaunchKernelGGL(k_shortlist<kMaxShort / 64>, g, b, 0, s, a);
+  if (a.L <= 64) {
+    hipLaunchKernelGGL(k_shortlist<1>, g, b, 0, s, a);
+  } else if (a.L <= 256) {
+    hipLaunchKernelGGL(k_shortlist<4>, g, b, 0, s, a);
+  } else {
+    if ((size_t)a.NL * kLowpLP > 256u * kSelPT) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_shortlist_sel, dim3(a.nq), b, 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -421,11 +555,21 @@ hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L,
 // for Euclidean, of the keys themselves).  Certified queries get their ids
 // and scores written; the others are listed for the FP64 scan.
 
+// Candidate rows reach the lanes through LDS: per round of 64 candidates
+// the wave copies kRC columns of all 64 rows at a time (lane l fetches
+// column l%16 of candidates 4u + l/16, u = 0..15: each load instruction
+// reads 4 contiguous 128-byte row pieces), then each lane continues its own
+// candidate's sequential fma chain over those columns.  The FP64 keys are
+// the canonical chains, in index order, as before.
+constexpr int kRC = 16;
+
 template <int METRIC>
 __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
+  __shared__ double stage[4][64 * (kRC + 1)];
   const int lane = threadIdx.x & 63;
   const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= a.nq) return;
+  double* stg = stage[threadIdx.x >> 6];
   // shortlist row i belongs to query q (its parameters, FP64 row, outputs)
   const uint32_t q = a.qidx ? a.qidx[i] : i;
   const QParams qp = a.qp[q];
@@ -445,18 +589,42 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
     else if (j < ncand) slot = a.degen[j - a.L];
     bool valid = slot != ~0u && slot < a.st.nrows &&
                  ((a.st.alive[slot >> 6] >> (slot & 63)) & 1ull);
-    uint64_t K = ~0ull;
-    if (valid) {
-      const double* __restrict__ x = a.st.X + (size_t)slot * a.st.ldx;
-      double acc = 0.0;
-      for (uint32_t c = 0; c < a.d; c++) {
-        if (METRIC == kL2) {
-          double t = x[c] - qv[c];
-          acc = fma(t, t, acc);
-        } else {
-          acc = fma(qv[c], x[c], acc);
+    // rows this lane fetches: candidates 4u + lane/16
+    const double* src[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int cand = u * 4 + (lane >> 4);
+      const uint32_t sl = __shfl(slot, cand);
+      const int ok = __shfl(valid ? 1 : 0, cand);
+      src[u] = ok ? a.st.X + (size_t)sl * a.st.ldx : nullptr;
+    }
+    double acc = 0.0;
+    const double* my = stg + lane * (kRC + 1);
+    for (uint32_t c0 = 0; c0 < a.d; c0 += kRC) {
+      const uint32_t e = c0 + (lane & 15);
+      double vals[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) vals[u] = (src[u] && e < a.d) ? src[u][e] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; u++) stg[(u * 4 + (lane >> 4)) * (kRC + 1) + (lane & 15)] = vals[u];
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t ne = min((uint32_t)kRC, a.d - c0);
+#pragma unroll
+      for (int t = 0; t < kRC; t++) {
+        if ((uint32_t)t < ne) {
+          const double x = my[t];
+          if (METRIC == kL2) {
+            const double tt = x - qv[c0 + t];
+            acc = fma(tt, tt, acc);
+          } else {
+            acc = fma(qv[c0 + t], x, acc);
+          }
         }
       }
+      __builtin_amdgcn_wave_barrier();
+    }
+    uint64_t K = ~0ull;
+    if (valid) {
       double v = acc;
       if (METRIC == kDot) v = -acc;
       else if (METRIC == kCos) v = -(acc / (qp.qn * a.st.xn[slot]));
