@@ -93,7 +93,7 @@ int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t di
 /* Engine knobs. */
 int svdb_set_shortlist(VectorDatabase* db, size_t width); /* first-pass shortlist L (>= k) */
 int svdb_set_budget_bytes(VectorDatabase* db, size_t bytes); /* device-memory budget */
-int svdb_enable_shadow(VectorDatabase* db, int precision); /* build the BF16 (d <= 256) or
+int svdb_enable_shadow(VectorDatabase* db, int precision); /* build the BF16 (d <= 768) or
                                                                FP8 (d <= 1536) shadow */
 
 typedef struct svdb_stats {
@@ -107,7 +107,8 @@ typedef struct svdb_stats {
     uint64_t certified;     /* low-precision queries certified exact */
     uint64_t fallbacks;     /* low-precision queries rescanned in FP64 */
     double last_max_rel_err;/* max |first-pass score - FP64 score| / |FP64| of the last call */
-    uint64_t widened;       /* of the certified: certified only by the 1024-wide retry */
+    uint64_t widened;       /* of the certified: certified only by a wider retry (1024, 4096) */
+    uint64_t second_pass;   /* FP8 queries handed to the BF16 first pass (counted again there) */
 } svdb_stats;
 int svdb_get_stats(VectorDatabase* db, svdb_stats* out);
 
@@ -117,7 +118,9 @@ typedef struct svdb_kernel_time {
     double total_ms;
 } svdb_kernel_time;
 enum { SVDB_K_SCAN_FP64 = 0, SVDB_K_SCAN_LOWP = 1, SVDB_K_SHORTLIST = 2, SVDB_K_RERANK = 3,
-       SVDB_K_MERGE = 4, SVDB_K_PREPASS = 5 /* sample scan + shortlist */, SVDB_K_COUNT = 6 };
+       SVDB_K_MERGE = 4, SVDB_K_PREPASS = 5 /* sample scan + k-th key */,
+       SVDB_K_SECOND_PASS = 6 /* FP8 queries rerun through the BF16 pass, whole */,
+       SVDB_K_COUNT = 7 };
 int svdb_set_profiling(VectorDatabase* db, int on); /* also resets the counters */
 int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_COUNT] */);
 

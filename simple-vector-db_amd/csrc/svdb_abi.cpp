@@ -614,6 +614,7 @@ int svdb_get_stats(VectorDatabase* db, svdb_stats* out) {
   out->fallbacks = s.fallbacks;
   out->last_max_rel_err = s.last_max_rel_err;
   out->widened = s.widened;
+  out->second_pass = s.second_pass;
   return SVDB_OK;
 }
 

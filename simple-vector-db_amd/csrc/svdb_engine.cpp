@@ -116,7 +116,8 @@ Engine::~Engine() {
   (void)hipStreamSynchronize(stream_);
   void* bufs[] = {X_, nx2_, xn_, alive_, rank_, s_q_, s_qn_, s_k_, s_pk_, s_ps_, s_ids_, s_sc_,
                   s_pair_, s_pout_, s_qb_, s_qp_, s_ck_, s_cs_, s_sk_, s_ss_, s_beta_, s_fail_,
-                  s_err_, s_fq_, s_fid_, s_fsc_, s_fk_, s_sk2_, s_ss2_, s_beta2_, s_fail2_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
+                  s_err_, s_fq_, s_fid_, s_fsc_, s_fk_, s_sk2_, s_ss2_, s_beta2_, s_fail2_,
+                  s_cidx_, s_cq_, s_cid_, s_csc_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
                   sh_.rho_bits /* degen_count and degen live inside this block */, sh8_.X8,
                   sh8_.sx, sh8_.nx, sh8_.xn, sh8_.inv, sh8_.pen, sh8_.rho_bits};
   for (void* p : bufs)
@@ -177,9 +178,10 @@ int Engine::ensure_dim(size_t d) {
     // 256 elements (the scan keeps a wavefront's queries in registers).
     size_t ks = 1;
     while (32 * ks < d) ks *= 2;
-    sh_.dp = 32 * ks;
+    // wider rows: the wide kernel's padded widths 512 and 768 (0: none)
+    sh_.dp = d <= 256 ? 32 * ks : d <= 512 ? 512 : d <= 768 ? 768 : 0;
     const char* off = std::getenv("SVDB_NO_SHADOW");
-    lowp_ = lowp_wanted_ && ks <= 8 && !(off && *off && *off != '0');
+    lowp_ = lowp_wanted_ && sh_.dp != 0 && !(off && *off && *off != '0');
     // FP8 rows: the smallest supported padded length (0: no FP8 first pass)
     for (size_t c : {256, 512, 768, 1024, 1536})
       if (d <= c) {
@@ -366,8 +368,8 @@ int Engine::enable_shadow(int precision) {
     return SVDB_OK;
   }
   if (fp8 ? lowp8_ : lowp_) return SVDB_OK;
-  if (!fp8 && sh_.dp > 256) {
-    set_error(SVDB_EINVAL, "BF16 first pass supports dimensions up to 256");
+  if (!fp8 && sh_.dp == 0) {
+    set_error(SVDB_EINVAL, "BF16 first pass supports dimensions up to 768");
     return SVDB_EINVAL;
   }
   if (fp8 && !fp8_dp_supported(sh8_.dp)) {
@@ -544,14 +546,14 @@ hipEvent_t Engine::take_event() {
 }
 
 void Engine::prof_begin(int which) {
-  if (!prof_) return;
+  if (!prof_ || prof_mute_) return;
   ProfSlot ps{which, take_event(), take_event()};
   (void)hipEventRecord(ps.a, stream_);
   prof_pending_.push_back(ps);
 }
 
 void Engine::prof_end(int which) {
-  if (!prof_) return;
+  if (!prof_ || prof_mute_) return;
   for (auto it = prof_pending_.rbegin(); it != prof_pending_.rend(); ++it)
     if (it->kind == which) {
       (void)hipEventRecord(it->b, stream_);
@@ -803,6 +805,43 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
     nf = *h_fail_;
     fail_list = out_list;
   }
+  // first-pass error report over the queries certified so far
+  std::vector<double> err(nq);
+  HIPCHK(hipMemcpy(err.data(), s_err_, nq * sizeof(double), hipMemcpyDeviceToHost), "copy");
+  double mx = 0.0;
+  for (double e : err) mx = std::max(mx, e);
+  st_.certified += nq - nf;
+  st_.widened += nf1 - nf;
+  // Second chance for FP8: queries its stages leave uncertified go through
+  // the BF16 first pass when the store has a BF16 shadow (a bound about 16x
+  // tighter); that pass rescans in FP64 only what it cannot certify.
+  const bool bf16_ok = lowp_ && (sh_.dp <= 256 || bf16_wide_dp_supported(sh_.dp));
+  if (nf > 0 && fp8 && bf16_ok) {
+    if ((rc = scratch(&s_cidx_, &s_cidx_n_, (size_t)nf * sizeof(uint32_t)))) return rc;
+    if ((rc = scratch(&s_cq_, &s_cq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
+    if ((rc = scratch(&s_cid_, &s_cid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
+    if ((rc = scratch(&s_csc_, &s_csc_n_, (size_t)nf * k * sizeof(double)))) return rc;
+    // the BF16 pass reuses the fail-list buffers: keep this list apart
+    HIPCHK(hipMemcpyAsync(s_cidx_, fail_list, nf * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                          stream_), "copy");
+    HIPCHK(launch_gather_rows(dQ, ldq, (const uint32_t*)s_cidx_, nf, (uint32_t)d_, (double*)s_cq_,
+                              d_, stream_), "k_gather_rows");
+    st_.second_pass += nf;
+    prof_begin(SVDB_K_SECOND_PASS);
+    prof_mute_ = true;
+    rc = run_lowp((const double*)s_cq_, d_, nf, metric, k, (uint64_t*)s_cid_, (double*)s_csc_,
+                  (uint32_t)k, false);
+    prof_mute_ = false;
+    if (rc) return rc;
+    prof_end(SVDB_K_SECOND_PASS);
+    HIPCHK(launch_scatter_results((const uint64_t*)s_cid_, (const double*)s_csc_, (uint32_t)k,
+                                  (const uint32_t*)s_cidx_, nf, dids, dscores, ostride, stream_),
+           "k_scatter_results");
+    HIPCHK(hipStreamSynchronize(stream_), "second pass");
+    st_.last_max_rel_err = std::max(mx, st_.last_max_rel_err);
+    return SVDB_OK;
+  }
+  st_.last_max_rel_err = mx;
   if (nf > 0) {
     if ((rc = scratch(&s_fq_, &s_fq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
     if ((rc = scratch(&s_fid_, &s_fid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
@@ -819,15 +858,8 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
                                   fail_list, nf, dids, dscores, ostride, stream_),
            "k_scatter_results");
     HIPCHK(hipStreamSynchronize(stream_), "fallback");
+    st_.fallbacks += nf;
   }
-  std::vector<double> err(nq);
-  HIPCHK(hipMemcpy(err.data(), s_err_, nq * sizeof(double), hipMemcpyDeviceToHost), "copy");
-  double mx = 0.0;
-  for (double e : err) mx = std::max(mx, e);
-  st_.last_max_rel_err = mx;
-  st_.certified += nq - nf;
-  st_.fallbacks += nf;
-  st_.widened += nf1 - nf;
   return SVDB_OK;
 }
 
@@ -836,7 +868,7 @@ int Engine::use_lowp(int precision, size_t D, int* rc) {
   if (precision == SVDB_PREC_FP64) return 0;
   const bool bf = lowp_ && D == d_, f8 = lowp8_ && D == d_;
   if (precision == SVDB_PREC_BF16 && !bf) {
-    set_error(SVDB_EINVAL, "BF16 first pass needs the shadow (dimension <= 256) and all columns");
+    set_error(SVDB_EINVAL, "BF16 first pass needs the shadow (dimension <= 768) and all columns");
     *rc = SVDB_EINVAL;
     return 0;
   }

@@ -55,7 +55,7 @@ struct KernelTime {
 };
 
 struct EngineStats {
-  uint64_t searches = 0, queries = 0, certified = 0, fallbacks = 0, widened = 0;
+  uint64_t searches = 0, queries = 0, certified = 0, fallbacks = 0, widened = 0, second_pass = 0;
   double last_max_rel_err = 0.0;
 };
 
@@ -182,6 +182,10 @@ class Engine {
   void* s_ss2_ = nullptr;    size_t s_ss2_n_ = 0;   //   slots
   void* s_beta2_ = nullptr;  size_t s_beta2_n_ = 0; //   bounds
   void* s_fail2_ = nullptr;  size_t s_fail2_n_ = 0; //   queries still uncertified
+  void* s_cidx_ = nullptr;   size_t s_cidx_n_ = 0;  // FP8 -> BF16 second pass: query indices
+  void* s_cq_ = nullptr;     size_t s_cq_n_ = 0;    //   gathered queries
+  void* s_cid_ = nullptr;    size_t s_cid_n_ = 0;   //   their ids
+  void* s_csc_ = nullptr;    size_t s_csc_n_ = 0;   //   and scores
   uint32_t* h_fail_ = nullptr;  // pinned: fail count
   void* h_stage_ = nullptr;     // pinned: row upload staging
   size_t h_stage_n_ = 0;
@@ -191,6 +195,7 @@ class Engine {
     hipEvent_t a, b;
   };
   bool prof_ = false;
+  bool prof_mute_ = false;  // inside the BF16 second pass: timed as a whole
   std::vector<ProfSlot> prof_pending_;
   std::vector<hipEvent_t> ev_pool_;
   KernelTime kt_[SVDB_K_COUNT];

@@ -154,7 +154,12 @@ hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices,
 // fewer than L are finite).  One wavefront per query.
 hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L, float* thr,
                           hipStream_t s);
-// ---- FP8 first pass (svdb_fp8.hip) ----
+// ---- wide rows (svdb_wide.hip): BF16 at padded widths 512 and 768 ----
+bool bf16_wide_dp_supported(size_t dp);
+hipError_t launch_scan_bf16_wide(const LowpScanArgs& a, int metric, uint32_t nslices,
+                                 bool minonly, hipStream_t s);
+
+// ---- FP8 first pass (svdb_wide.hip) ----
 // FP8 rows are padded to a multiple of 256 bytes; supported padded lengths.
 bool fp8_dp_supported(size_t dp);
 int fp8_queries_per_block(size_t dp);

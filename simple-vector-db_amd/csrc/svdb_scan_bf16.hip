@@ -258,11 +258,15 @@ static hipError_t scan_bf16_m(const LowpScanArgs& a, uint32_t nslices, bool mino
   return hipErrorInvalidValue;
 }
 
-int lowp_queries_per_block(size_t dp) { return dp / 32 == 8 ? kLowpWaves * 2 * 16 : kLowpQPB; }
+int lowp_queries_per_block(size_t dp) {
+  if (dp > 256) return kLowpWaves * 16;  // k_scan_wide, one fragment per wavefront
+  return dp / 32 == 8 ? kLowpWaves * 2 * 16 : kLowpQPB;
+}
 
 hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices, bool minonly,
                             hipStream_t s) {
   if (a.nq == 0 || nslices == 0) return hipSuccess;
+  if (a.dp > 256) return launch_scan_bf16_wide(a, metric, nslices, minonly, s);
   switch (metric) {
     case kL2: return scan_bf16_m<kL2>(a, nslices, minonly, s);
     case kCos: return scan_bf16_m<kCos>(a, nslices, minonly, s);

@@ -1,5 +1,6 @@
-// svdb_fp8.hip — FP8 (OCP e4m3fn) shadow and first pass for wide rows
-// (gfx950).  Built with -ffp-contract=off; every fma below is explicit.
+// svdb_wide.hip — FP8 (OCP e4m3fn) shadow, and the first passes over wide
+// rows: FP8 (padded widths 256..1536) and BF16 (512, 768) (gfx950).  Built
+// with -ffp-contract=off; every fma below is explicit.
 //
 // Rows are stored as e4m3 codes of x / sx with a per-row power-of-two scale
 // sx (max |x| / sx in (224, 448]), so x̂ = sx · e4m3(x / sx) exactly and
@@ -7,7 +8,7 @@
 // key: the bound of svdb_lowp.h holds with this shadow's ρ (≈ 2^-5 instead
 // of ≈ 2^-9).  Queries are quantized the same way, with their own scale sq.
 //
-// Scan: the structure of k_scan_bf16 (8 wavefronts, queries held as B
+// Scan (k_scan_wide): the structure of k_scan_bf16 (8 wavefronts, queries held as B
 // operands, row tiles streamed direct-to-LDS into two static buffers, XCD
 // owning whole slices, per-lane 8-entry sorted lists) with
 // v_mfma_scale_f32_16x16x128_f8f6f4 (unit E8M0 scales): lane l supplies
@@ -26,6 +27,7 @@ namespace svdb {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((address_space(3))) void lds_void;
 
 // ------------------------------------------------------------- e4m3
@@ -304,7 +306,7 @@ __device__ __forceinline__ void issue_tile8(const LowpScanArgs& a, unsigned char
     const int p = p0 + lane;
     const int row = p / G::CPR, pos = p % G::CPR;
     const int c = pos ^ (row & 7);
-    load_lds16(X8 + (size_t)(row0 + row) * a.dp + c * 16, buf + p0 * 16);
+    load_lds16(X8 + (size_t)(row0 + row) * G::RB + c * 16, buf + p0 * 16);
   }
   // per-row values: waves 0-1 w0, waves 2-3 w1, 64 rows per wave
   float* aux = reinterpret_cast<float*>(buf + G::TILE_B);
@@ -317,8 +319,11 @@ __device__ __forceinline__ void issue_tile8(const LowpScanArgs& a, unsigned char
   }
 }
 
-template <int METRIC, int KS, int QF, bool MINONLY>
-__global__ __launch_bounds__(512) void k_scan_fp8(LowpScanArgs a) {
+// BF: BF16 rows (64 elements per 128-byte K step, two
+// v_mfma_f32_16x16x32_bf16 per step: lane l gives elements 8·(l/16)..+7 of
+// each 32-element half; one-FMA keys as in k_scan_bf16); else FP8.
+template <int METRIC, int KS, int QF, bool MINONLY, bool BF>
+__global__ __launch_bounds__(512) void k_scan_wide(LowpScanArgs a) {
   using G = Geom8<KS>;
   __shared__ __attribute__((aligned(16))) unsigned char buf0[G::BUF_B];
   __shared__ __attribute__((aligned(16))) unsigned char buf1[G::BUF_B];
@@ -339,6 +344,8 @@ __global__ __launch_bounds__(512) void k_scan_fp8(LowpScanArgs a) {
   const bool active = qbase < a.nq;  // wave-uniform
 
   const uint8_t* Q8 = reinterpret_cast<const uint8_t*>(a.Qb);
+  // B fragments: FP8 bytes 128·ks + 32·rg .. +31; BF16 the two 16-byte
+  // pieces at 128·ks + 16·rg and 128·ks + 64 + 16·rg
   i32x8 B[QF][KS];
   float P[QF], tau[QF];
   float lk[QF][LP];
@@ -346,9 +353,17 @@ __global__ __launch_bounds__(512) void k_scan_fp8(LowpScanArgs a) {
 #pragma unroll
   for (int f = 0; f < QF; f++) {
     const uint32_t q = qbase + f * 16 + ql;  // < nq_pad (Q8 is padded)
-    const uint8_t* qrow = Q8 + (size_t)q * a.dp + rg * 32;
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) B[f][ks] = *reinterpret_cast<const i32x8*>(qrow + ks * 128);
+    for (int ks = 0; ks < KS; ks++) {
+      if constexpr (BF) {
+        const uint8_t* qrow = Q8 + (size_t)q * G::RB + rg * 16 + ks * 128;
+        const i32x4 lo = *reinterpret_cast<const i32x4*>(qrow);
+        const i32x4 hi = *reinterpret_cast<const i32x4*>(qrow + 64);
+        B[f][ks] = i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      } else {
+        B[f][ks] = *reinterpret_cast<const i32x8*>(Q8 + (size_t)q * G::RB + rg * 32 + ks * 128);
+      }
+    }
     P[f] = a.qp[min(q, a.nq - 1)].P;
     float t0 = -INFINITY;
     if (q < a.nq) {
@@ -374,7 +389,9 @@ __global__ __launch_bounds__(512) void k_scan_fp8(LowpScanArgs a) {
       float k4[4];
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        k4[r] = COS ? fmaf(acc[f][r] * P[f], v0[r], v1[r]) : fmaf(P[f], v0[r], acc[f][r] * v1[r]);
+        k4[r] = BF ? (COS ? fmaf(acc[f][r], v0[r], v1[r]) : fmaf(P[f], v0[r], acc[f][r]))
+                   : (COS ? fmaf(acc[f][r] * P[f], v0[r], v1[r])
+                          : fmaf(P[f], v0[r], acc[f][r] * v1[r]));
       const float mn = fminf(fminf(k4[0], k4[1]), fminf(k4[2], k4[3]));
       if constexpr (MINONLY) {
         tau[f] = fminf(tau[f], mn);
@@ -418,9 +435,10 @@ __global__ __launch_bounds__(512) void k_scan_fp8(LowpScanArgs a) {
       // step ks's MFMAs; the scheduling barrier keeps the other steps'
       // reads from being hoisted, which would spill the 96 B registers.
       auto loadA = [&](int arow, int ks) {
-        const int c0 = ks * 8 + rg * 2;
+        const int c0 = BF ? ks * 8 + rg : ks * 8 + rg * 2;
+        const int c1 = BF ? c0 + 4 : c0 + 1;
         const i32x4 lo = *reinterpret_cast<const i32x4*>(cur + G::chunk(arow, c0) * 16);
-        const i32x4 hi = *reinterpret_cast<const i32x4*>(cur + G::chunk(arow, c0 + 1) * 16);
+        const i32x4 hi = *reinterpret_cast<const i32x4*>(cur + G::chunk(arow, c1) * 16);
         return i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
       };
 #pragma unroll
@@ -435,9 +453,20 @@ __global__ __launch_bounds__(512) void k_scan_fp8(LowpScanArgs a) {
           i32x8 An = A;
           if (ks + 1 < KS) An = loadA(arow, ks + 1);
 #pragma unroll
-          for (int f = 0; f < QF; f++)
-            acc[f] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(A, B[f][ks], acc[f], 0, 0, 0,
-                                                                      127, 0, 127);
+          for (int f = 0; f < QF; f++) {
+            if constexpr (BF) {
+              const i32x8 b = B[f][ks];
+              const bf16x8 a0 = __builtin_bit_cast(bf16x8, i32x4{A[0], A[1], A[2], A[3]});
+              const bf16x8 a1 = __builtin_bit_cast(bf16x8, i32x4{A[4], A[5], A[6], A[7]});
+              const bf16x8 b0 = __builtin_bit_cast(bf16x8, i32x4{b[0], b[1], b[2], b[3]});
+              const bf16x8 b1 = __builtin_bit_cast(bf16x8, i32x4{b[4], b[5], b[6], b[7]});
+              acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[f], 0, 0, 0);
+              acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[f], 0, 0, 0);
+            } else {
+              acc[f] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(A, B[f][ks], acc[f], 0, 0,
+                                                                        0, 127, 0, 127);
+            }
+          }
           A = An;
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -468,10 +497,10 @@ __global__ __launch_bounds__(512) void k_scan_fp8(LowpScanArgs a) {
   }
 }
 
-template <int METRIC, int KS, int QF>
+template <int METRIC, int KS, int QF, bool BF = false>
 static hipError_t scan_fp8_t(const LowpScanArgs& a, uint32_t nslices, bool minonly,
                              hipStream_t s) {
-  auto fn = minonly ? k_scan_fp8<METRIC, KS, QF, true> : k_scan_fp8<METRIC, KS, QF, false>;
+  auto fn = minonly ? k_scan_wide<METRIC, KS, QF, true, BF> : k_scan_wide<METRIC, KS, QF, false, BF>;
   const uint32_t qpb = kLowpWaves * QF * 16;
   // nslices is a multiple of 8 (one set of slices per XCD); rows per slice
   // and the slice stride are multiples of the 128-row slot granularity.
@@ -491,6 +520,29 @@ static hipError_t scan_fp8_m(const LowpScanArgs& a, uint32_t nslices, bool minon
     case 768: return scan_fp8_t<METRIC, 6, 2>(a, nslices, minonly, s);
     case 1024: return scan_fp8_t<METRIC, 8, 1>(a, nslices, minonly, s);
     case 1536: return scan_fp8_t<METRIC, 12, 1>(a, nslices, minonly, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int METRIC>
+static hipError_t scan_bf16w_m(const LowpScanArgs& a, uint32_t nslices, bool minonly,
+                               hipStream_t s) {
+  switch (a.dp) {
+    case 512: return scan_fp8_t<METRIC, 8, 1, true>(a, nslices, minonly, s);
+    case 768: return scan_fp8_t<METRIC, 12, 1, true>(a, nslices, minonly, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+bool bf16_wide_dp_supported(size_t dp) { return dp == 512 || dp == 768; }
+
+hipError_t launch_scan_bf16_wide(const LowpScanArgs& a, int metric, uint32_t nslices,
+                                 bool minonly, hipStream_t s) {
+  if (a.nq == 0 || nslices == 0) return hipSuccess;
+  switch (metric) {
+    case kL2: return scan_bf16w_m<kL2>(a, nslices, minonly, s);
+    case kCos: return scan_bf16w_m<kCos>(a, nslices, minonly, s);
+    case kDot: return scan_bf16w_m<kDot>(a, nslices, minonly, s);
   }
   return hipErrorInvalidValue;
 }

@@ -17,7 +17,7 @@ L2, COS, DOT = 0, 1, 2
 METRICS = {"l2": L2, "euclidean": L2, "cos": COS, "cosine": COS, "dot": DOT}
 PREC_FP64, PREC_BF16, PREC_FP8, PREC_AUTO = 0, 1, 2, 3
 PRECISIONS = {"fp64": PREC_FP64, "bf16": PREC_BF16, "fp8": PREC_FP8, "auto": PREC_AUTO}
-K_SCAN_FP64, K_SCAN_LOWP, K_SHORTLIST, K_RERANK, K_MERGE, K_PREPASS, K_COUNT = range(7)
+K_SCAN_FP64, K_SCAN_LOWP, K_SHORTLIST, K_RERANK, K_MERGE, K_PREPASS, K_SECOND_PASS, K_COUNT = range(8)
 UUID_SIZE = 37
 MAX_K = 128
 NOID = np.uint64(2**64 - 1)
@@ -58,7 +58,7 @@ class Stats(C.Structure):
                 ("device_bytes", C.c_size_t), ("budget_bytes", C.c_size_t),
                 ("searches", C.c_uint64), ("queries", C.c_uint64), ("certified", C.c_uint64),
                 ("fallbacks", C.c_uint64), ("last_max_rel_err", C.c_double),
-                ("widened", C.c_uint64)]
+                ("widened", C.c_uint64), ("second_pass", C.c_uint64)]
 
 
 class KernelTime(C.Structure):

@@ -69,8 +69,9 @@ def test_fp8_exact_768(wide768, metric, k):
     cert = s1["certified"] - s0["certified"]
     fb = s1["fallbacks"] - s0["fallbacks"]
     wid = s1["widened"] - s0["widened"]
-    say(f"  certified {cert} ({wid} by the 1024-wide retry), fp64 rescans {fb}, "
-        f"first-pass err {s1['last_max_rel_err']:.3g}")
+    sec = s1["second_pass"] - s0["second_pass"]
+    say(f"  certified {cert} ({wid} by a wider retry), {sec} handed to the BF16 pass, "
+        f"fp64 rescans {fb}, first-pass err {s1['last_max_rel_err']:.3g}")
     assert cert + fb == Q.shape[0]
     if k <= 10:
         assert cert >= 0.5 * Q.shape[0]
@@ -151,4 +152,47 @@ def test_fp8_enable_errors():
         db.enable_shadow("fp8")  # wider than 1536
     with pytest.raises(svdb.SvdbError):
         db.search(np.zeros((1, 2000)), 1, "l2", precision="fp8")
+    db.close()
+
+
+@pytest.mark.parametrize("metric", ["l2", "cos", "dot"])
+def test_bf16_wide_768(wide768, metric):
+    """The BF16 first pass at padded width 768 (k_scan_wide, BF16 operands)."""
+    X, Q, db = wide768
+    for k in (10, 100):
+        say(f"bf16 768 {metric} k={k}")
+        s0 = db.stats()
+        ids, sc = db.search(Q, k, metric, precision="bf16")
+        s1 = db.stats()
+        rid, rsc = O.topk(X, Q, k, metric)
+        assert_same(ids, sc, rid, rsc, f"bf16 768 {metric} k={k}")
+        cert = s1["certified"] - s0["certified"]
+        say(f"  certified {cert} of {Q.shape[0]}, first-pass err {s1['last_max_rel_err']:.3g}")
+        assert cert >= 0.9 * Q.shape[0]
+        assert 1e-7 < s1["last_max_rel_err"] < 5e-2
+
+
+def test_bf16_wide_512_mutations():
+    rng = np.random.default_rng(12)
+    d = 400  # padded to 512
+    X = workloads.mixture(30_000, d, seed=57, n_centers=16, lognorm=0.5)
+    db = store(X, fp8=False)
+    rows = [r for r in X]
+    for i in (2, 900):
+        v = rng.standard_normal(d)
+        db.update(i, v)
+        rows[i] = v
+    db.delete(17)
+    del rows[17]
+    for v in (np.zeros(d), rng.standard_normal(d)):
+        db.insert(v)
+        rows.append(v)
+    Xm = np.stack(rows)
+    Q = np.vstack([Xm[[1, 500]] + 1e-3, workloads.mixture(30, d, seed=58, n_centers=16)])
+    for metric in ("l2", "cos", "dot"):
+        say(f"bf16 512 mutations {metric}")
+        ids, sc = db.search(Q, 10, metric, precision="auto")
+        rid, rsc = O.topk(Xm, Q, 10, metric)
+        assert_same(ids, sc, rid, rsc, f"bf16 512 {metric}")
+    assert db.stats()["certified"] > 0
     db.close()
