@@ -93,8 +93,8 @@ int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t di
 /* Engine knobs. */
 int svdb_set_shortlist(VectorDatabase* db, size_t width); /* first-pass shortlist L (>= k) */
 int svdb_set_budget_bytes(VectorDatabase* db, size_t bytes); /* device-memory budget */
-int svdb_enable_shadow(VectorDatabase* db, int precision); /* build the BF16 (d <= 768) or
-                                                               FP8 (d <= 1536) shadow */
+int svdb_enable_shadow(VectorDatabase* db, int precision); /* build the BF16 or FP8 shadow
+                                                               (d <= 1536) */
 
 typedef struct svdb_stats {
     size_t rows;            /* logical rows */

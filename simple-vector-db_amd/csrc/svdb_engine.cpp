@@ -178,8 +178,8 @@ int Engine::ensure_dim(size_t d) {
     // 256 elements (the scan keeps a wavefront's queries in registers).
     size_t ks = 1;
     while (32 * ks < d) ks *= 2;
-    // wider rows: the wide kernel's padded widths 512 and 768 (0: none)
-    sh_.dp = d <= 256 ? 32 * ks : d <= 512 ? 512 : d <= 768 ? 768 : 0;
+    // wider rows: the wide kernel's padded widths 512, 768, 1536 (0: none)
+    sh_.dp = d <= 256 ? 32 * ks : d <= 512 ? 512 : d <= 768 ? 768 : d <= 1536 ? 1536 : 0;
     const char* off = std::getenv("SVDB_NO_SHADOW");
     lowp_ = lowp_wanted_ && sh_.dp != 0 && !(off && *off && *off != '0');
     // FP8 rows: the smallest supported padded length (0: no FP8 first pass)
@@ -369,7 +369,7 @@ int Engine::enable_shadow(int precision) {
   }
   if (fp8 ? lowp8_ : lowp_) return SVDB_OK;
   if (!fp8 && sh_.dp == 0) {
-    set_error(SVDB_EINVAL, "BF16 first pass supports dimensions up to 768");
+    set_error(SVDB_EINVAL, "BF16 first pass supports dimensions up to 1536");
     return SVDB_EINVAL;
   }
   if (fp8 && !fp8_dp_supported(sh8_.dp)) {
@@ -868,7 +868,7 @@ int Engine::use_lowp(int precision, size_t D, int* rc) {
   if (precision == SVDB_PREC_FP64) return 0;
   const bool bf = lowp_ && D == d_, f8 = lowp8_ && D == d_;
   if (precision == SVDB_PREC_BF16 && !bf) {
-    set_error(SVDB_EINVAL, "BF16 first pass needs the shadow (dimension <= 768) and all columns");
+    set_error(SVDB_EINVAL, "BF16 first pass needs the shadow (dimension <= 1536) and all columns");
     *rc = SVDB_EINVAL;
     return 0;
   }

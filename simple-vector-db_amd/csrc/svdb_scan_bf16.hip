@@ -259,6 +259,7 @@ static hipError_t scan_bf16_m(const LowpScanArgs& a, uint32_t nslices, bool mino
 }
 
 int lowp_queries_per_block(size_t dp) {
+  if (dp > 768) return 4 * 16;           // k_scan_wide, 4 wavefronts of one fragment
   if (dp > 256) return kLowpWaves * 16;  // k_scan_wide, one fragment per wavefront
   return dp / 32 == 8 ? kLowpWaves * 2 * 16 : kLowpQPB;
 }

@@ -1,5 +1,5 @@
 // svdb_wide.hip — FP8 (OCP e4m3fn) shadow, and the first passes over wide
-// rows: FP8 (padded widths 256..1536) and BF16 (512, 768) (gfx950).  Built
+// rows: FP8 (padded widths 256..1536) and BF16 (512, 768, 1536) (gfx950).  Built
 // with -ffp-contract=off; every fma below is explicit.
 //
 // Rows are stored as e4m3 codes of x / sx with a per-row power-of-two scale
@@ -276,18 +276,19 @@ __device__ __forceinline__ void load_lds4(const void* g, void* lds_base) {
   __builtin_amdgcn_global_load_lds(g, (lds_void*)lds_base, 4, 0, 0);
 }
 
-// KS K-steps of 128 bytes per row.  Tiles of TR rows (TR divides 128, the
-// store's slot granularity) of at most 48 KB, two per block.
-template <int KS>
+// KS K-steps of 128 bytes per row, NW wavefronts per block.  Tiles of TR
+// rows (TR divides 128, the store's slot granularity) of at most 48 KB, two
+// per block.
+template <int KS, int NW = kLowpWaves>
 struct Geom8 {
   static constexpr int RB = KS * 128;  // row bytes
-  static constexpr int TR = KS <= 2 ? 128 : (KS <= 6 ? 64 : 32);
+  static constexpr int TR = KS <= 2 ? 128 : (KS <= 6 ? 64 : (KS <= 12 ? 32 : 16));
   static constexpr int CPR = RB / 16;  // 16-byte chunks per row
   static constexpr int TILE_B = TR * RB;
   static constexpr int AUX_B = TR * 4 * 2;
   static constexpr int BUF_B = TILE_B + AUX_B;
-  static constexpr int CPT = TR * CPR / (64 * kLowpWaves);  // chunk loads per lane
-  static_assert(TR * CPR % (64 * kLowpWaves) == 0, "tile chunks must split over the block");
+  static constexpr int CPT = TR * CPR / (64 * NW);  // chunk loads per lane
+  static_assert(TR * CPR % (64 * NW) == 0, "tile chunks must split over the block");
   static_assert(BUF_B * 2 <= 160 * 1024, "two tiles must fit LDS");
   // LDS chunk of (row, column chunk c): XOR-swizzled within aligned groups
   // of 8 chunks, so 8 consecutive rows of one column chunk hit 8 distinct
@@ -295,14 +296,14 @@ struct Geom8 {
   __device__ static __forceinline__ int chunk(int row, int c) { return row * CPR + (c ^ (row & 7)); }
 };
 
-template <int KS>
+template <int KS, int NW>
 __device__ __forceinline__ void issue_tile8(const LowpScanArgs& a, unsigned char* buf,
                                             uint32_t row0, int wave, int lane) {
-  using G = Geom8<KS>;
+  using G = Geom8<KS, NW>;
   const uint8_t* X8 = reinterpret_cast<const uint8_t*>(a.Xb);
 #pragma unroll
   for (int j = 0; j < G::CPT; j++) {
-    const int p0 = (wave + kLowpWaves * j) * 64;  // first LDS chunk of this instruction
+    const int p0 = (wave + NW * j) * 64;  // first LDS chunk of this instruction
     const int p = p0 + lane;
     const int row = p / G::CPR, pos = p % G::CPR;
     const int c = pos ^ (row & 7);
@@ -322,9 +323,11 @@ __device__ __forceinline__ void issue_tile8(const LowpScanArgs& a, unsigned char
 // BF: BF16 rows (64 elements per 128-byte K step, two
 // v_mfma_f32_16x16x32_bf16 per step: lane l gives elements 8·(l/16)..+7 of
 // each 32-element half; one-FMA keys as in k_scan_bf16); else FP8.
-template <int METRIC, int KS, int QF, bool MINONLY, bool BF>
-__global__ __launch_bounds__(512) void k_scan_wide(LowpScanArgs a) {
-  using G = Geom8<KS>;
+// NW = 4 (one wavefront per SIMD) leaves each wavefront 512 registers, for
+// the 192-register query fragment of a 1536-wide BF16 row.
+template <int METRIC, int KS, int QF, bool MINONLY, bool BF, int NW>
+__global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
+  using G = Geom8<KS, NW>;
   __shared__ __attribute__((aligned(16))) unsigned char buf0[G::BUF_B];
   __shared__ __attribute__((aligned(16))) unsigned char buf1[G::BUF_B];
   constexpr int LP = kLowpLP;
@@ -333,7 +336,7 @@ __global__ __launch_bounds__(512) void k_scan_wide(LowpScanArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int ql = lane & 15, rg = lane >> 4;
-  const uint32_t qpb = kLowpWaves * QF * 16;
+  const uint32_t qpb = NW * QF * 16;
   const uint32_t nqb = (a.nq + qpb - 1) / qpb;
   const uint32_t xcd = blockIdx.x % 8, local = blockIdx.x / 8;
   const uint32_t qbase = (local % nqb) * qpb + wave * QF * 16;
@@ -418,7 +421,7 @@ __global__ __launch_bounds__(512) void k_scan_wide(LowpScanArgs a) {
     }
   };
 
-  if (ntiles > 0) issue_tile8<KS>(a, buf0, r0, wave, lane);
+  if (ntiles > 0) issue_tile8<KS, NW>(a, buf0, r0, wave, lane);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (uint32_t t0 = 0; t0 < ntiles; t0 += 2) {
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(512) void k_scan_wide(LowpScanArgs a) {
       const uint32_t t = t0 + h;
       if (t >= ntiles) break;
       const unsigned char* cur = h ? buf1 : buf0;
-      if (t + 1 < ntiles) issue_tile8<KS>(a, h ? buf0 : buf1, r0 + (t + 1) * G::TR, wave, lane);
+      if (t + 1 < ntiles) issue_tile8<KS, NW>(a, h ? buf0 : buf1, r0 + (t + 1) * G::TR, wave, lane);
       const uint32_t trow = r0 + t * G::TR;
       const float* aux = reinterpret_cast<const float*>(cur + G::TILE_B);
       // A operand of K step ks: bytes 128·ks + 32·rg .. +31 of row arow
@@ -497,17 +500,18 @@ __global__ __launch_bounds__(512) void k_scan_wide(LowpScanArgs a) {
   }
 }
 
-template <int METRIC, int KS, int QF, bool BF = false>
+template <int METRIC, int KS, int QF, bool BF = false, int NW = kLowpWaves>
 static hipError_t scan_fp8_t(const LowpScanArgs& a, uint32_t nslices, bool minonly,
                              hipStream_t s) {
-  auto fn = minonly ? k_scan_wide<METRIC, KS, QF, true, BF> : k_scan_wide<METRIC, KS, QF, false, BF>;
-  const uint32_t qpb = kLowpWaves * QF * 16;
+  auto fn = minonly ? k_scan_wide<METRIC, KS, QF, true, BF, NW>
+                    : k_scan_wide<METRIC, KS, QF, false, BF, NW>;
+  const uint32_t qpb = NW * QF * 16;
   // nslices is a multiple of 8 (one set of slices per XCD); rows per slice
   // and the slice stride are multiples of the 128-row slot granularity.
-  if (nslices % 8 || a.rows_per_slice % Geom8<KS>::TR || a.nrows_pad % 128)
+  if (nslices % 8 || a.rows_per_slice % Geom8<KS, NW>::TR || a.nrows_pad % 128)
     return hipErrorInvalidValue;
   dim3 grid(((a.nq + qpb - 1) / qpb) * nslices);
-  hipLaunchKernelGGL(fn, grid, dim3(512), 0, s, a);
+  hipLaunchKernelGGL(fn, grid, dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 
@@ -530,11 +534,12 @@ static hipError_t scan_bf16w_m(const LowpScanArgs& a, uint32_t nslices, bool min
   switch (a.dp) {
     case 512: return scan_fp8_t<METRIC, 8, 1, true>(a, nslices, minonly, s);
     case 768: return scan_fp8_t<METRIC, 12, 1, true>(a, nslices, minonly, s);
+    case 1536: return scan_fp8_t<METRIC, 24, 1, true, 4>(a, nslices, minonly, s);
   }
   return hipErrorInvalidValue;
 }
 
-bool bf16_wide_dp_supported(size_t dp) { return dp == 512 || dp == 768; }
+bool bf16_wide_dp_supported(size_t dp) { return dp == 512 || dp == 768 || dp == 1536; }
 
 hipError_t launch_scan_bf16_wide(const LowpScanArgs& a, int metric, uint32_t nslices,
                                  bool minonly, hipStream_t s) {

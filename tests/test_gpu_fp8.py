@@ -89,12 +89,17 @@ def test_fp8_1536_and_auto():
         ids, sc = db.search(Q, 10, metric, precision="fp8")
         rid, rsc = O.topk(X, Q, 10, metric)
         assert_same(ids, sc, rid, rsc, f"fp8 1536 {metric}")
-    # no BF16 shadow at d > 256: "auto" takes the FP8 first pass
-    s0 = db.stats()
-    ids, sc = db.search(Q, 10, "dot", precision="auto")
-    assert db.stats()["certified"] + db.stats()["fallbacks"] > s0["certified"] + s0["fallbacks"]
-    rid, rsc = O.topk(X, Q, 10, "dot")
-    assert_same(ids, sc, rid, rsc, "fp8 1536 auto")
+    # the BF16 first pass at padded width 1536 (4 wavefronts per block), and
+    # "auto" (BF16 when the store has that shadow)
+    for metric in ("l2", "cos", "dot"):
+        for prec in ("bf16", "auto"):
+            say(f"{prec} 1536 {metric}")
+            s0 = db.stats()
+            ids, sc = db.search(Q, 10, metric, precision=prec)
+            s1 = db.stats()
+            rid, rsc = O.topk(X, Q, 10, metric)
+            assert_same(ids, sc, rid, rsc, f"{prec} 1536 {metric}")
+            assert s1["certified"] - s0["certified"] >= 0.9 * Q.shape[0]
     db.close()
 
 
