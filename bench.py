@@ -74,6 +74,8 @@ def serve(srv, plan, pool, rate=None):
     lat_us = np.zeros(n, dtype=np.int64)
     tick_q = np.zeros(n, dtype=np.int64)
     tick_us = np.zeros(n, dtype=np.int64)
+    lag_us = np.zeros(n, dtype=np.int64)
+    queue_us = np.zeros(n, dtype=np.int64)
     nqs = np.array([p[2] if p[0] == "search" else 0 for p in plan])
     ready = threading.Semaphore(0)
     errors = []
@@ -106,6 +108,8 @@ def serve(srv, plan, pool, rate=None):
             lat_us[i] = info["done_us"] - sched[i]
             tick_q[i] = info["tick_queries"]
             tick_us[i] = info["done_us"] - info["start_us"]
+            lag_us[i] = info["submit_us"] - sched[i]
+            queue_us[i] = info["start_us"] - info["submit_us"]
 
     ts = time.perf_counter()
     th = [threading.Thread(target=submitter), threading.Thread(target=waiter)]
@@ -124,6 +128,9 @@ def serve(srv, plan, pool, rate=None):
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_tick_queries": round(float(tick_q[srch].mean()), 1),
             "mean_tick_ms": round(float(tick_us[srch].mean()) * 1e-3, 3),
+            "p99_tick_ms": round(float(np.percentile(tick_us[srch], 99)) * 1e-3, 3),
+            "p99_queue_ms": round(float(np.percentile(queue_us[srch], 99)) * 1e-3, 3),
+            "p99_submit_lag_ms": round(float(np.percentile(lag_us[srch], 99)) * 1e-3, 3),
             "mutation_ms": round(float(tick_us[~srch].mean()) * 1e-3, 3) if (~srch).any() else None}
 
 

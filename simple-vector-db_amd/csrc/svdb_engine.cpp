@@ -653,7 +653,29 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
 // gathered and answered by the FP64 scan.
 int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k,
                      uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
+  LowpRange r{metric, 0, nq, 0};
+  return run_lowp_ranges(dQ, ldq, &r, 1, k, dids, dscores, ostride, fp8);
+}
+
+// Low-precision search of nr metric ranges: range i holds R[i].n queries of
+// metric R[i].metric at query rows R[i].qoff.. (dQ, dids, dscores).  With
+// several ranges (BF16 rows <= 256 wide only) one mixed-metric first pass
+// serves them all: each range takes a run of slots starting on a wavefront
+// boundary (padding slots are flagged and never scanned); the pre-pass,
+// rerank, widening and fallback then run per range.
+int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, int k,
+                            uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
   int rc;
+  const bool mixed = nr > 1;
+  if (mixed && (fp8 || sh_.dp > 256)) {
+    set_error(SVDB_EINVAL, "mixed-metric first pass needs the BF16 shadow of rows <= 256 wide");
+    return SVDB_EINVAL;
+  }
+  uint32_t nq = 0;  // slots
+  for (int i = 0; i < nr; i++) {
+    R[i].soff = nq;
+    nq += mixed ? (uint32_t)round_up(R[i].n, kSlotAlign) : R[i].n;
+  }
   const size_t dp = fp8 ? sh8_.dp : sh_.dp;
   const size_t qbytes = fp8 ? 1 : sizeof(uint16_t);  // bytes per query element
   const uint32_t qpb = (uint32_t)(fp8 ? fp8_queries_per_block(dp) : lowp_queries_per_block(dp));
@@ -661,7 +683,7 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   // Shortlist: the configured width (64 by default) for small k; for BF16
   // 256 once k passes 32, where the k-th and L-th neighbours need more room
   // than the first-pass error bound leaves in a 2k-wide list.  FP8 starts at
-  // the configured width (or k) and relies on the 1024-wide retry below.
+  // the configured width (or k) and relies on the wider retries below.
   const uint32_t L = (uint32_t)std::min<size_t>(
       kMaxShort, std::max<size_t>({shortlist_, (size_t)k, (!fp8 && k > 32) ? (size_t)256 : 0}));
   const uint32_t nrows_pad = (uint32_t)round_up(nphys_, kLowpTile);
@@ -672,8 +694,8 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   const uint32_t tiles = nrows_pad / kLowpTile;
   uint32_t ns = std::max<uint32_t>(8, (512 + groups - 1) / groups);
   // Past k = 32 the lists' own bound (the smallest 8th entry) must stay
-  // well beyond the k-th neighbour: at least 256 lists per query.
-  if (k > 32) ns = std::max<uint32_t>(ns, 64);
+  // well beyond the k-th neighbour: at least 512 lists per query.
+  if (k > 32) ns = std::max<uint32_t>(ns, 128);
   // FP8: the widening retries take up to kMaxShort candidates from these
   // lists; 2048 lists per query keep their own bound (the smallest 8th
   // entry) clear of the 4096th candidate.
@@ -709,39 +731,58 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   if (!h_fail_) HIPCHK(hipHostMalloc((void**)&h_fail_, sizeof(uint32_t)), "hipHostMalloc");
   if ((rc = sync_alive())) return rc;
 
-  uint32_t* fail_count = (uint32_t*)s_fail_;
-  uint32_t* fail_list = fail_count + 1;  // may be redirected to the widening retry's list
-  HIPCHK(hipMemsetAsync(fail_count, 0, sizeof(uint32_t), stream_), "memset");
   HIPCHK(hipMemsetAsync(s_err_, 0, (size_t)nq * sizeof(double), stream_), "memset");
-  if (fp8)
-    HIPCHK(launch_query_prep8(dQ, ldq, nq, nq_pad, (uint32_t)d_, dp, metric, sh8_.rho_bits,
-                              (uint8_t*)s_qb_, (QParams*)s_qp_, stream_), "k_query_prep8");
-  else
-    HIPCHK(launch_query_prep(dQ, ldq, nq, nq_pad, (uint32_t)d_, dp, metric, sh_.rho_bits,
-                             (uint16_t*)s_qb_, (QParams*)s_qp_, stream_), "k_query_prep");
+  // query prep per range; each range's slots run to the next range (the last
+  // to nq_pad), the slots past its queries flagged as padding
+  for (int i = 0; i < nr; i++) {
+    const uint32_t send = i + 1 < nr ? R[i + 1].soff : nq_pad;
+    const double* q = dQ + (size_t)R[i].qoff * ldq;
+    unsigned char* qb = (unsigned char*)s_qb_ + (size_t)R[i].soff * dp * qbytes;
+    QParams* qp = (QParams*)s_qp_ + R[i].soff;
+    if (fp8)
+      HIPCHK(launch_query_prep8(q, ldq, R[i].n, send - R[i].soff, (uint32_t)d_, dp, R[i].metric,
+                                sh8_.rho_bits, (uint8_t*)qb, qp, stream_), "k_query_prep8");
+    else
+      HIPCHK(launch_query_prep(q, ldq, R[i].n, send - R[i].soff, (uint32_t)d_, dp, R[i].metric,
+                               sh_.rho_bits, (uint16_t*)qb, qp, stream_), "k_query_prep");
+  }
   // per-row ordering-key arrays: BF16 L2 (nx, pen), dot (xn, pen), cosine
   // (inv, pen); FP8 L2 (nx, sx), dot (xn, sx), cosine (sx/‖x‖, pen)
-  const float* w0 = fp8 ? (metric == kL2 ? sh8_.nx : metric == kDot ? sh8_.xn : sh8_.inv)
-                        : (metric == kL2 ? sh_.nx : metric == kDot ? sh_.xn : sh_.inv);
-  const float* w1 = fp8 ? (metric == kCos ? sh8_.pen : sh8_.sx) : sh_.pen;
+  auto w0_of = [&](int m) {
+    return fp8 ? (m == kL2 ? sh8_.nx : m == kDot ? sh8_.xn : sh8_.inv)
+               : (m == kL2 ? sh_.nx : m == kDot ? sh_.xn : sh_.inv);
+  };
+  auto w1_of = [&](int m) { return fp8 ? (m == kCos ? sh8_.pen : sh8_.sx) : sh_.pen; };
   const uint32_t* degen = fp8 ? sh8_.degen : sh_.degen;
   const uint32_t* degen_count = fp8 ? sh8_.degen_count : sh_.degen_count;
-  auto scan = [&](const LowpScanArgs& x, uint32_t nsl, bool minonly) {
-    return fp8 ? launch_scan_fp8(x, metric, nsl, minonly, stream_)
-               : launch_scan_bf16(x, metric, nsl, minonly, stream_);
+  auto scan = [&](const LowpScanArgs& x, int m, uint32_t nsl, bool minonly) {
+    return fp8 ? launch_scan_fp8(x, m, nsl, minonly, stream_)
+               : launch_scan_bf16(x, m, nsl, minonly, stream_);
   };
-  LowpScanArgs sa{fp8 ? (const uint16_t*)sh8_.X8 : sh_.Xb, dp, w0, w1, (const uint16_t*)s_qb_,
-                  (const QParams*)s_qp_, nq, rps, rps, nrows_pad, NL, (float*)s_ck_,
-                  (uint32_t*)s_cs_, nullptr, 0};
+  const int smetric = mixed ? kMixed : R[0].metric;
+  LowpScanArgs sa{fp8 ? (const uint16_t*)sh8_.X8 : sh_.Xb, dp,
+                  mixed ? sh_.nx : w0_of(R[0].metric), mixed ? sh_.pen : w1_of(R[0].metric),
+                  mixed ? sh_.xn : nullptr, mixed ? sh_.inv : nullptr,
+                  (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, rps, rps, nrows_pad, NL,
+                  (float*)s_ck_, (uint32_t*)s_cs_, nullptr, 0};
   if (pre) {
-    LowpScanArgs pa = sa;
-    pa.rows_per_slice = tp * kLowpTile;
-    pa.slice_stride = (tiles / ns_pre) * kLowpTile;
-    pa.NL = NL_pre;
     prof_begin(SVDB_K_PREPASS);
-    HIPCHK(scan(pa, ns_pre, true), "first-pass scan (sample)");
+    for (int i = 0; i < nr; i++) {
+      LowpScanArgs pa = sa;
+      pa.w0 = w0_of(R[i].metric);
+      pa.w1 = w1_of(R[i].metric);
+      pa.w2 = pa.w3 = nullptr;
+      pa.Qb = (const uint16_t*)((const unsigned char*)s_qb_ + (size_t)R[i].soff * dp * qbytes);
+      pa.qp = (const QParams*)s_qp_ + R[i].soff;
+      pa.nq = R[i].n;
+      pa.rows_per_slice = tp * kLowpTile;
+      pa.slice_stride = (tiles / ns_pre) * kLowpTile;
+      pa.NL = NL_pre;
+      pa.cand_key = (float*)s_ck_ + (size_t)R[i].soff * NL_pre;
+      HIPCHK(scan(pa, R[i].metric, ns_pre, true), "first-pass scan (sample)");
+    }
     // thresholds in s_beta_, read by the main scan before the shortlist
-    // below overwrites them
+    // below overwrites them (padding slots' values are never used)
     HIPCHK(launch_kth_key((const float*)s_ck_, nq, NL_pre, L, (float*)s_beta_, stream_),
            "k_kth_key");
     prof_end(SVDB_K_PREPASS);
@@ -749,7 +790,7 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
     sa.tau0_stride = 1;
   }
   prof_begin(SVDB_K_SCAN_LOWP);
-  HIPCHK(scan(sa, ns, false), "first-pass scan");
+  HIPCHK(scan(sa, smetric, ns, false), "first-pass scan");
   prof_end(SVDB_K_SCAN_LOWP);
   ShortlistArgs sl{(const float*)s_ck_, (const uint32_t*)s_cs_, nq, NL, L, (uint32_t*)s_ss_,
                    (float*)s_sk_, (float*)s_beta_};
@@ -757,109 +798,127 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
   HIPCHK(launch_shortlist(sl, stream_), "k_shortlist");
   prof_end(SVDB_K_SHORTLIST);
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
-  RerankArgs ra{sv, dQ, ldq, (uint32_t)d_, (const QParams*)s_qp_, (const uint32_t*)s_ss_,
-                (const float*)s_sk_, (const float*)s_beta_, degen, degen_count, w0, w1,
-                nq, L, (uint32_t)k, ostride, dids, dscores, fail_count, fail_list,
-                (double*)s_err_};
-  prof_begin(SVDB_K_RERANK);
-  HIPCHK(launch_rerank(ra, metric, stream_), "k_rerank");
-  prof_end(SVDB_K_RERANK);
-  HIPCHK(hipMemcpyAsync(h_fail_, fail_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "copy");
-  HIPCHK(hipStreamSynchronize(stream_), "lowp");
-  const uint32_t nf1 = *h_fail_;
-  uint32_t nf = nf1;
-  // Widening retries: queries not certified at L are shortlisted again at
-  // 1024, then 4096, from the same candidate lists (when they hold at least
-  // twice that many entries) and reranked; only those still uncertified are
-  // rescanned in FP64.
-  for (uint32_t Lw : {1024u, (uint32_t)kMaxShort}) {
-    if (nf == 0 || Lw <= L || (size_t)NL * kLowpLP < 2 * (size_t)Lw) continue;
-    if ((rc = scratch(&s_sk2_, &s_sk2_n_, (size_t)nf * Lw * sizeof(float)))) return rc;
-    if ((rc = scratch(&s_ss2_, &s_ss2_n_, (size_t)nf * Lw * sizeof(uint32_t)))) return rc;
-    if ((rc = scratch(&s_beta2_, &s_beta2_n_, (size_t)nf * sizeof(float)))) return rc;
-    if ((rc = scratch(&s_fail2_, &s_fail2_n_, 2 * ((size_t)nf + 1) * sizeof(uint32_t)))) return rc;
-    // two fail lists alternate: this stage reads `fail_list`, writes the other
-    uint32_t* out_count = (uint32_t*)s_fail2_ + (fail_list == (uint32_t*)s_fail2_ + 1 ? nf + 1 : 0);
-    uint32_t* out_list = out_count + 1;
-    HIPCHK(hipMemsetAsync(out_count, 0, sizeof(uint32_t), stream_), "memset");
-    ShortlistArgs sw{(const float*)s_ck_, (const uint32_t*)s_cs_, nf, NL, Lw, (uint32_t*)s_ss2_,
-                     (float*)s_sk2_, (float*)s_beta2_, fail_list};
-    prof_begin(SVDB_K_SHORTLIST);
-    HIPCHK(launch_shortlist(sw, stream_), "k_shortlist (widened)");
-    prof_end(SVDB_K_SHORTLIST);
-    RerankArgs rw = ra;
-    rw.short_slot = (const uint32_t*)s_ss2_;
-    rw.short_key = (const float*)s_sk2_;
-    rw.beta = (const float*)s_beta2_;
-    rw.nq = nf;
-    rw.L = Lw;
-    rw.fail_count = out_count;
-    rw.fail_list = out_list;
-    rw.qidx = fail_list;
-    prof_begin(SVDB_K_RERANK);
-    HIPCHK(launch_rerank(rw, metric, stream_), "k_rerank (widened)");
-    prof_end(SVDB_K_RERANK);
-    HIPCHK(hipMemcpyAsync(h_fail_, out_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
-           "copy");
-    HIPCHK(hipStreamSynchronize(stream_), "lowp widened");
-    nf = *h_fail_;
-    fail_list = out_list;
-  }
-  // first-pass error report over the queries certified so far
-  std::vector<double> err(nq);
-  HIPCHK(hipMemcpy(err.data(), s_err_, nq * sizeof(double), hipMemcpyDeviceToHost), "copy");
   double mx = 0.0;
-  for (double e : err) mx = std::max(mx, e);
-  st_.certified += nq - nf;
-  st_.widened += nf1 - nf;
-  // Second chance for FP8: queries its stages leave uncertified go through
-  // the BF16 first pass when the store has a BF16 shadow (a bound about 16x
-  // tighter); that pass rescans in FP64 only what it cannot certify.
-  const bool bf16_ok = lowp_ && (sh_.dp <= 256 || bf16_wide_dp_supported(sh_.dp));
-  if (nf > 0 && fp8 && bf16_ok) {
-    if ((rc = scratch(&s_cidx_, &s_cidx_n_, (size_t)nf * sizeof(uint32_t)))) return rc;
-    if ((rc = scratch(&s_cq_, &s_cq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
-    if ((rc = scratch(&s_cid_, &s_cid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
-    if ((rc = scratch(&s_csc_, &s_csc_n_, (size_t)nf * k * sizeof(double)))) return rc;
-    // the BF16 pass reuses the fail-list buffers: keep this list apart
-    HIPCHK(hipMemcpyAsync(s_cidx_, fail_list, nf * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                          stream_), "copy");
-    HIPCHK(launch_gather_rows(dQ, ldq, (const uint32_t*)s_cidx_, nf, (uint32_t)d_, (double*)s_cq_,
-                              d_, stream_), "k_gather_rows");
-    st_.second_pass += nf;
-    prof_begin(SVDB_K_SECOND_PASS);
-    prof_mute_ = true;
-    rc = run_lowp((const double*)s_cq_, d_, nf, metric, k, (uint64_t*)s_cid_, (double*)s_csc_,
-                  (uint32_t)k, false);
-    prof_mute_ = false;
-    if (rc) return rc;
-    prof_end(SVDB_K_SECOND_PASS);
-    HIPCHK(launch_scatter_results((const uint64_t*)s_cid_, (const double*)s_csc_, (uint32_t)k,
-                                  (const uint32_t*)s_cidx_, nf, dids, dscores, ostride, stream_),
-           "k_scatter_results");
-    HIPCHK(hipStreamSynchronize(stream_), "second pass");
-    st_.last_max_rel_err = std::max(mx, st_.last_max_rel_err);
-    return SVDB_OK;
-  }
-  st_.last_max_rel_err = mx;
-  if (nf > 0) {
+  for (int ri = 0; ri < nr; ri++) {
+    const LowpRange& rg = R[ri];
+    const int metric = rg.metric;
+    const uint32_t nqr = rg.n;
+    if (nqr == 0) continue;
+    // this range's views: slot-space arrays at soff, query-space at qoff
+    const double* rQ = dQ + (size_t)rg.qoff * ldq;
+    uint64_t* rids = dids + (size_t)rg.qoff * ostride;
+    double* rsc = dscores + (size_t)rg.qoff * ostride;
+    const float* rck = (const float*)s_ck_ + (size_t)rg.soff * NL * kLowpLP;
+    const uint32_t* rcs = (const uint32_t*)s_cs_ + (size_t)rg.soff * NL * kLowpLP;
+    uint32_t* fail_count = (uint32_t*)s_fail_;
+    uint32_t* fail_list = fail_count + 1;  // may be redirected to a widening retry's list
+    HIPCHK(hipMemsetAsync(fail_count, 0, sizeof(uint32_t), stream_), "memset");
+    RerankArgs ra{sv, rQ, ldq, (uint32_t)d_, (const QParams*)s_qp_ + rg.soff,
+                  (const uint32_t*)s_ss_ + (size_t)rg.soff * L,
+                  (const float*)s_sk_ + (size_t)rg.soff * L, (const float*)s_beta_ + rg.soff,
+                  degen, degen_count, w0_of(metric), w1_of(metric), nqr, L, (uint32_t)k,
+                  ostride, rids, rsc, fail_count, fail_list, (double*)s_err_ + rg.soff};
+    prof_begin(SVDB_K_RERANK);
+    HIPCHK(launch_rerank(ra, metric, stream_), "k_rerank");
+    prof_end(SVDB_K_RERANK);
+    HIPCHK(hipMemcpyAsync(h_fail_, fail_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "copy");
+    HIPCHK(hipStreamSynchronize(stream_), "lowp");
+    const uint32_t nf1 = *h_fail_;
+    uint32_t nf = nf1;
+    // Widening retries: queries not certified at L are shortlisted again at
+    // 1024, then 4096, from the same candidate lists (when they hold at
+    // least twice that many entries) and reranked; only those still
+    // uncertified go on.
+    for (uint32_t Lw : {1024u, (uint32_t)kMaxShort}) {
+      if (nf == 0 || Lw <= L || (size_t)NL * kLowpLP < 2 * (size_t)Lw) continue;
+      if ((rc = scratch(&s_sk2_, &s_sk2_n_, (size_t)nf * Lw * sizeof(float)))) return rc;
+      if ((rc = scratch(&s_ss2_, &s_ss2_n_, (size_t)nf * Lw * sizeof(uint32_t)))) return rc;
+      if ((rc = scratch(&s_beta2_, &s_beta2_n_, (size_t)nf * sizeof(float)))) return rc;
+      if ((rc = scratch(&s_fail2_, &s_fail2_n_, 2 * ((size_t)nf + 1) * sizeof(uint32_t)))) return rc;
+      // two fail lists alternate: this stage reads `fail_list`, writes the other
+      uint32_t* out_count =
+          (uint32_t*)s_fail2_ + (fail_list == (uint32_t*)s_fail2_ + 1 ? nf + 1 : 0);
+      uint32_t* out_list = out_count + 1;
+      HIPCHK(hipMemsetAsync(out_count, 0, sizeof(uint32_t), stream_), "memset");
+      ShortlistArgs sw{rck, rcs, nf, NL, Lw, (uint32_t*)s_ss2_, (float*)s_sk2_,
+                       (float*)s_beta2_, fail_list};
+      prof_begin(SVDB_K_SHORTLIST);
+      HIPCHK(launch_shortlist(sw, stream_), "k_shortlist (widened)");
+      prof_end(SVDB_K_SHORTLIST);
+      RerankArgs rw = ra;
+      rw.short_slot = (const uint32_t*)s_ss2_;
+      rw.short_key = (const float*)s_sk2_;
+      rw.beta = (const float*)s_beta2_;
+      rw.nq = nf;
+      rw.L = Lw;
+      rw.fail_count = out_count;
+      rw.fail_list = out_list;
+      rw.qidx = fail_list;
+      prof_begin(SVDB_K_RERANK);
+      HIPCHK(launch_rerank(rw, metric, stream_), "k_rerank (widened)");
+      prof_end(SVDB_K_RERANK);
+      HIPCHK(hipMemcpyAsync(h_fail_, out_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+             "copy");
+      HIPCHK(hipStreamSynchronize(stream_), "lowp widened");
+      nf = *h_fail_;
+      fail_list = out_list;
+    }
+    // first-pass error report over the queries certified so far
+    std::vector<double> err(nqr);
+    HIPCHK(hipMemcpy(err.data(), (double*)s_err_ + rg.soff, nqr * sizeof(double),
+                     hipMemcpyDeviceToHost), "copy");
+    for (double e : err) mx = std::max(mx, e);
+    st_.certified += nqr - nf;
+    st_.widened += nf1 - nf;
+    if (nf == 0) continue;
+    // Second chance for FP8: queries its stages leave uncertified go through
+    // the BF16 first pass when the store has a BF16 shadow (a bound about
+    // 16x tighter); that pass rescans in FP64 only what it cannot certify.
+    const bool bf16_ok = lowp_ && (sh_.dp <= 256 || bf16_wide_dp_supported(sh_.dp));
+    if (fp8 && bf16_ok) {
+      if ((rc = scratch(&s_cidx_, &s_cidx_n_, (size_t)nf * sizeof(uint32_t)))) return rc;
+      if ((rc = scratch(&s_cq_, &s_cq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
+      if ((rc = scratch(&s_cid_, &s_cid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
+      if ((rc = scratch(&s_csc_, &s_csc_n_, (size_t)nf * k * sizeof(double)))) return rc;
+      // the BF16 pass reuses the fail-list buffers: keep this list apart
+      HIPCHK(hipMemcpyAsync(s_cidx_, fail_list, nf * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                            stream_), "copy");
+      HIPCHK(launch_gather_rows(rQ, ldq, (const uint32_t*)s_cidx_, nf, (uint32_t)d_,
+                                (double*)s_cq_, d_, stream_), "k_gather_rows");
+      st_.second_pass += nf;
+      prof_begin(SVDB_K_SECOND_PASS);
+      prof_mute_ = true;
+      const double keep = mx;
+      rc = run_lowp((const double*)s_cq_, d_, nf, metric, k, (uint64_t*)s_cid_, (double*)s_csc_,
+                    (uint32_t)k, false);
+      prof_mute_ = false;
+      if (rc) return rc;
+      prof_end(SVDB_K_SECOND_PASS);
+      mx = std::max(keep, st_.last_max_rel_err);
+      HIPCHK(launch_scatter_results((const uint64_t*)s_cid_, (const double*)s_csc_, (uint32_t)k,
+                                    (const uint32_t*)s_cidx_, nf, rids, rsc, ostride, stream_),
+             "k_scatter_results");
+      HIPCHK(hipStreamSynchronize(stream_), "second pass");
+      continue;
+    }
     if ((rc = scratch(&s_fq_, &s_fq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
     if ((rc = scratch(&s_fid_, &s_fid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
     if ((rc = scratch(&s_fsc_, &s_fsc_n_, (size_t)nf * k * sizeof(double)))) return rc;
     if ((rc = scratch(&s_fk_, &s_fk_n_, (size_t)nf * sizeof(int)))) return rc;
     std::vector<int> hk(nf, k);
     HIPCHK(hipMemcpyAsync(s_fk_, hk.data(), nf * sizeof(int), hipMemcpyHostToDevice, stream_), "copy");
-    HIPCHK(launch_gather_rows(dQ, ldq, fail_list, nf, (uint32_t)d_, (double*)s_fq_, d_, stream_),
+    HIPCHK(launch_gather_rows(rQ, ldq, fail_list, nf, (uint32_t)d_, (double*)s_fq_, d_, stream_),
            "k_gather_rows");
     rc = run_exact((const double*)s_fq_, d_, nf, (uint32_t)d_, (const int*)s_fk_, metric, k,
                    (uint64_t*)s_fid_, (double*)s_fsc_, (uint32_t)k);
     if (rc) return rc;
     HIPCHK(launch_scatter_results((const uint64_t*)s_fid_, (const double*)s_fsc_, (uint32_t)k,
-                                  fail_list, nf, dids, dscores, ostride, stream_),
+                                  fail_list, nf, rids, rsc, ostride, stream_),
            "k_scatter_results");
     HIPCHK(hipStreamSynchronize(stream_), "fallback");
     st_.fallbacks += nf;
   }
+  st_.last_max_rel_err = mx;
   return SVDB_OK;
 }
 
@@ -981,6 +1040,49 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   std::vector<uint64_t> hid;
   std::vector<double> hsc;
   std::vector<size_t> sel;
+  // Several metrics in one BF16 batch (rows <= 256 wide): one mixed-metric
+  // first pass for all of them, queries grouped by metric.
+  if (which == 1 && sh_.dp <= 256 && nq <= step) {
+    size_t cnt[3] = {0, 0, 0};
+    for (size_t i = 0; i < nq; i++) cnt[metric[i]]++;
+    if ((cnt[0] > 0) + (cnt[1] > 0) + (cnt[2] > 0) >= 2) {
+      LowpRange R[3];
+      int nr = 0;
+      sel.clear();
+      for (int m = 0; m < 3; m++) {
+        if (!cnt[m]) continue;
+        R[nr++] = LowpRange{m, (uint32_t)sel.size(), (uint32_t)cnt[m], 0};
+        for (size_t i = 0; i < nq; i++)
+          if (metric[i] == m) sel.push_back(i);
+      }
+      const size_t n = sel.size();
+      hq.assign(n * D, 0.0);
+      for (size_t t = 0; t < n; t++) std::memcpy(&hq[t * D], Q + sel[t] * ldq, D * sizeof(double));
+      if ((rc = scratch(&s_q_, &s_q_n_, n * D * sizeof(double)))) return rc;
+      if ((rc = scratch(&s_ids_, &s_ids_n_, n * kmax_all * sizeof(uint64_t)))) return rc;
+      if ((rc = scratch(&s_sc_, &s_sc_n_, n * kmax_all * sizeof(double)))) return rc;
+      HIPCHK(hipMemcpyAsync(s_q_, hq.data(), n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
+      rc = run_lowp_ranges((const double*)s_q_, D, R, nr, (int)kmax_all, (uint64_t*)s_ids_,
+                           (double*)s_sc_, (uint32_t)kmax_all, false);
+      if (rc) return rc;
+      hid.resize(n * kmax_all);
+      hsc.resize(n * kmax_all);
+      HIPCHK(hipMemcpyAsync(hid.data(), s_ids_, n * kmax_all * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
+      HIPCHK(hipMemcpyAsync(hsc.data(), s_sc_, n * kmax_all * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
+      HIPCHK(hipStreamSynchronize(stream_), "search");
+      prof_collect();
+      for (size_t t = 0; t < n; t++) {
+        const size_t i = sel[t];
+        for (size_t j = 0; j < k[i]; j++) {
+          ids[i * ostride + j] = (size_t)hid[t * kmax_all + j];
+          scores[i * ostride + j] = hsc[t * kmax_all + j];
+        }
+      }
+      st_.searches++;
+      st_.queries += nq;
+      return SVDB_OK;
+    }
+  }
   for (int m = 0; m < 3; m++) {
     sel.clear();
     for (size_t i = 0; i < nq; i++)

@@ -111,8 +111,15 @@ class Engine {
   int scratch(void** p, size_t* have, size_t need);
   int run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk, int metric,
                 int kmax, uint64_t* dids, double* dscores, uint32_t ostride);
+  struct LowpRange {
+    int metric;
+    uint32_t qoff, n;  // query rows qoff .. qoff+n-1
+    uint32_t soff;     // first slot (set by run_lowp_ranges)
+  };
   int run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k, uint64_t* dids,
                double* dscores, uint32_t ostride, bool fp8);
+  int run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, int k, uint64_t* dids,
+                      double* dscores, uint32_t ostride, bool fp8);
   // 0: FP64 scan, 1: BF16 first pass, 2: FP8 first pass
   int use_lowp(int precision, size_t D, int* rc);
   int alloc_shadow(size_t cap);

@@ -35,6 +35,8 @@ constexpr int kLowpLP = 8;                         // entries per per-lane list
 constexpr int kLowpLanesPerQuery = 4;              // lists per query per slice
 constexpr int kDegenCap = 64;                      // degenerate rows kept in the list
 constexpr int kMaxShort = 4096;                    // widest shortlist (the widening retries)
+constexpr int kMixed = 3;                          // scan "metric": per-wavefront metric from QParams
+constexpr int kSlotAlign = 64;                     // metric ranges start on wavefront boundaries
 
 struct Shadow {
   uint16_t* Xb = nullptr;   // [cap][dp]
@@ -73,11 +75,12 @@ struct Shadow8 {
 // Per-query parameters written by the prep kernel.
 struct QParams {
   float P;        // epilogue multiplier (L2: a_q; dot: -c'_q; cos: unused)
-  int flags;      // 1 = degenerate query: FP64 only
+  int flags;      // 1 = degenerate query or padding slot: FP64 only, never scanned
   double scale;   // R >= O*scale + off
   double off;
   double qn;      // FP64 ‖q‖ (canonical chain)
   double nq2;     // FP64 ‖q‖²
+  int metric;     // kL2 / kCos / kDot (the first pass's key for this slot)
 };
 
 struct LowpScanArgs {
@@ -85,6 +88,8 @@ struct LowpScanArgs {
   size_t dp;
   const float* w0;       // per-row array for the key (nx / xn / inv)
   const float* w1;       // cosine: pen
+  const float* w2;       // mixed-metric scan only: xn (dot)
+  const float* w3;       //                         inv (cosine)
   const uint16_t* Qb;    // [nq_pad][dp]
   const QParams* qp;
   uint32_t nq;

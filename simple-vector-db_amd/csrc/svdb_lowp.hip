@@ -222,6 +222,7 @@ __global__ void k_query_prep(const double* __restrict__ Q, size_t ldq, uint32_t 
   if (i >= nq_pad) return;
   uint16_t* qb = Qb + (size_t)i * dp;
   QParams p{};
+  p.metric = metric;
   if (i >= nq) {
     for (size_t c = 0; c < dp; c++) qb[c] = 0;
     p.flags = 1;

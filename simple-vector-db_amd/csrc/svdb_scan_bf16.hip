@@ -34,11 +34,11 @@ __device__ __forceinline__ void load_lds4(const void* g, void* lds_base) {
   __builtin_amdgcn_global_load_lds(g, (lds_void*)lds_base, 4, 0, 0);
 }
 
-template <int KS>
+template <int KS, int NAUX = 2>
 struct TileGeom {
   static constexpr int CPR = 4 * KS;                 // 16-byte chunks per row
   static constexpr int TILE_B = kLowpTile * CPR * 16;
-  static constexpr int AUX_B = kLowpTile * 4 * 2;    // two fp32 arrays
+  static constexpr int AUX_B = kLowpTile * 4 * NAUX; // NAUX fp32 arrays
   static constexpr int BUF_B = TILE_B + AUX_B;
   static constexpr int CPW = kLowpTile * CPR / (64 * kLowpWaves);  // chunk-instrs per wave
   // LDS chunk of (row, column chunk c): swizzled within the row.
@@ -48,10 +48,12 @@ struct TileGeom {
 };
 
 // Issue the direct-to-LDS loads of one tile (rows row0..row0+127) into buf.
-template <int KS, bool TWO_AUX>
+// NAUX per-row arrays follow the tile: w0, w1 (cosine), and in the mixed
+// scan w2, w3; waves 2j, 2j+1 load array j.
+template <int KS, int NAUX>
 __device__ __forceinline__ void issue_tile(const LowpScanArgs& a, unsigned char* buf,
                                            uint32_t row0, int wave, int lane) {
-  using G = TileGeom<KS>;
+  using G = TileGeom<KS, NAUX>;
 #pragma unroll
   for (int j = 0; j < G::CPW; j++) {
     const int p0 = (wave + kLowpWaves * j) * 64;  // first LDS chunk of this instruction
@@ -61,17 +63,24 @@ __device__ __forceinline__ void issue_tile(const LowpScanArgs& a, unsigned char*
     load_lds16(a.Xb + (size_t)(row0 + row) * a.dp + c * 8, buf + p0 * 16);
   }
   float* aux = reinterpret_cast<float*>(buf + G::TILE_B);
-  if (wave < 2) load_lds4(a.w0 + row0 + wave * 64 + lane, aux + wave * 64);
-  else if (TWO_AUX && wave < 4)
-    load_lds4(a.w1 + row0 + (wave - 2) * 64 + lane, aux + kLowpTile + (wave - 2) * 64);
+  const int arr = wave >> 1, part = wave & 1;
+  if (arr < NAUX) {
+    const float* w = arr == 0 ? a.w0 : arr == 1 ? a.w1 : arr == 2 ? a.w2 : a.w3;
+    load_lds4(w + row0 + part * 64 + lane, aux + arr * kLowpTile + part * 64);
+  }
 }
 
 // MINONLY: each lane keeps only the smallest key of its rows (one list
 // entry, cand_key[q * NL + list]); the sample pre-pass uses it to bound the
 // L-th key cheaply.
+// METRIC kMixed: each wavefront's queries share one metric (QParams.metric
+// of its first slot; the engine aligns metric ranges to wavefronts), read at
+// run time; the tile then carries all four per-row arrays.
 template <int METRIC, int KS, int QF, bool MINONLY>
 __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
-  using G = TileGeom<KS>;
+  constexpr bool MIXED = METRIC == kMixed;
+  constexpr int NAUX = MIXED ? 4 : (METRIC == kCos ? 2 : 1);
+  using G = TileGeom<KS, NAUX < 2 ? 2 : NAUX>;
   __shared__ __attribute__((aligned(16))) unsigned char buf0[G::BUF_B];
   __shared__ __attribute__((aligned(16))) unsigned char buf1[G::BUF_B];
   constexpr int LP = kLowpLP;
@@ -93,6 +102,11 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   const uint32_t r1 = min(r0 + a.rows_per_slice, a.nrows_pad);
   const uint32_t ntiles = r1 > r0 ? (r1 - r0) / kLowpTile : 0;
   const bool active = qbase < a.nq;  // wave-uniform
+  // mixed scan: this wavefront's metric, and where its per-row array sits
+  // in the tile's aux block (0 nx, 2 xn, 3 inv; 1 is pen)
+  const int wm = MIXED ? a.qp[min(qbase, a.nq - 1)].metric : METRIC;
+  const bool wcos = MIXED ? wm == kCos : COS;
+  const int warr = MIXED ? (wm == kL2 ? 0 : wm == kDot ? 2 : 3) : 0;
 
   bf16x8 B[QF][KS];
   float P[QF], tau[QF];
@@ -104,11 +118,13 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
     const uint16_t* qrow = a.Qb + (size_t)q * a.dp + rg * 8;
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) B[f][ks] = *reinterpret_cast<const bf16x8*>(qrow + ks * 32);
-    P[f] = a.qp[min(q, a.nq - 1)].P;
-    // Padding lanes (q >= nq) never insert; real ones start from the
+    const QParams& qq = a.qp[min(q, a.nq - 1)];
+    P[f] = qq.P;
+    // Padding lanes (q >= nq, or a flagged slot: a degenerate query or the
+    // padding between metric ranges) never insert; real ones start from the
     // optional threshold (a NaN threshold counts as none).
     float t0 = -INFINITY;
-    if (q < a.nq) {
+    if (q < a.nq && !qq.flags) {
       t0 = INFINITY;
       if (a.tau0) {
         const float t = a.tau0[(size_t)q * a.tau0_stride];
@@ -129,19 +145,43 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   uint32_t prow = 0;
   bool pending = false;
 
+  auto key = [&](const f32x4 (&acc)[QF], int f, int r, const float (&wv)[4],
+                 const float (&pv)[4]) {
+    return MIXED ? fmaf(wcos ? acc[f][r] : P[f], wv[r], wcos ? pv[r] : acc[f][r])
+                 : (COS ? fmaf(acc[f][r], wv[r], pv[r]) : fmaf(P[f], wv[r], acc[f][r]));
+  };
+  // One test per subtile for all fragments: in the common case no key
+  // beats its list's 8th entry and the insertion code is skipped with a
+  // single branch; the rare insertion path recomputes the keys.
   auto epilogue = [&](const f32x4 (&acc)[QF], const float4& w, const float4& pen, uint32_t srow) {
     const float wv[4] = {w.x, w.y, w.z, w.w};
     const float pv[4] = {pen.x, pen.y, pen.z, pen.w};
+    if constexpr (MINONLY) {
+#pragma unroll
+      for (int f = 0; f < QF; f++) {
+        const float mn = fminf(fminf(key(acc, f, 0, wv, pv), key(acc, f, 1, wv, pv)),
+                               fminf(key(acc, f, 2, wv, pv), key(acc, f, 3, wv, pv)));
+        tau[f] = fminf(tau[f], mn);
+      }
+      return;
+    }
+    if constexpr (!MIXED) {  // (the mixed scan has no registers to spare for it)
+      bool need = false;
+#pragma unroll
+      for (int f = 0; f < QF; f++) {
+        const float mn = fminf(fminf(key(acc, f, 0, wv, pv), key(acc, f, 1, wv, pv)),
+                               fminf(key(acc, f, 2, wv, pv), key(acc, f, 3, wv, pv)));
+        need |= mn < tau[f];
+      }
+      if (!need) return;
+    }
 #pragma unroll
     for (int f = 0; f < QF; f++) {
       float k4[4];
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        k4[r] = COS ? fmaf(acc[f][r], wv[r], pv[r]) : fmaf(P[f], wv[r], acc[f][r]);
+      for (int r = 0; r < 4; r++) k4[r] = key(acc, f, r, wv, pv);
       const float mn = fminf(fminf(k4[0], k4[1]), fminf(k4[2], k4[3]));
-      if constexpr (MINONLY) {
-        tau[f] = fminf(tau[f], mn);
-      } else if (mn < tau[f]) {
+      if (mn < tau[f]) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           if (k4[r] < lk[f][LP - 1]) {
@@ -167,7 +207,7 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   // Two LDS buffers as distinct objects, picked at compile time (tiles are
   // taken in pairs and the pair loop is unrolled): reads of one buffer then
   // need not wait for the direct-to-LDS loads still filling the other.
-  if (ntiles > 0) issue_tile<KS, COS>(a, buf0, r0, wave, lane);
+  if (ntiles > 0) issue_tile<KS, NAUX>(a, buf0, r0, wave, lane);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (uint32_t t0 = 0; t0 < ntiles; t0 += 2) {
@@ -177,7 +217,7 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
       if (t >= ntiles) break;
       const unsigned char* cur = h ? buf1 : buf0;
       if (t + 1 < ntiles)
-        issue_tile<KS, COS>(a, h ? buf0 : buf1, r0 + (t + 1) * kLowpTile, wave, lane);
+        issue_tile<KS, NAUX>(a, h ? buf0 : buf1, r0 + (t + 1) * kLowpTile, wave, lane);
       const uint32_t trow = r0 + t * kLowpTile;
       const float* aux = reinterpret_cast<const float*>(cur + G::TILE_B);
       // A wave whose queries all lie past nq (a small batch) only helps
@@ -190,9 +230,9 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
         for (int ks = 0; ks < KS; ks++)
           A[ks] = *reinterpret_cast<const bf16x8*>(cur + G::chunk(arow, ks * 4 + rg) * 16);
         const int rl = sub * 16 + rg * 4;
-        const float4 w = *reinterpret_cast<const float4*>(aux + rl);
+        const float4 w = *reinterpret_cast<const float4*>(aux + warr * kLowpTile + rl);
         float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (COS) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+        if (wcos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
         f32x4 acc[QF];
 #pragma unroll
         for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -236,7 +276,14 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
 template <int METRIC, int KS, int QF>
 static hipError_t scan_bf16_t(const LowpScanArgs& a, uint32_t nslices, bool minonly,
                               hipStream_t s) {
-  auto fn = minonly ? k_scan_bf16<METRIC, KS, QF, true> : k_scan_bf16<METRIC, KS, QF, false>;
+  // the mixed scan has no pre-pass form: the engine runs the pre-pass per
+  // metric range with the single-metric kernels
+  if constexpr (METRIC == kMixed) {
+    if (minonly) return hipErrorInvalidValue;
+  }
+  auto fn = k_scan_bf16<METRIC, KS, QF, false>;
+  if constexpr (METRIC != kMixed)
+    if (minonly) fn = k_scan_bf16<METRIC, KS, QF, true>;
   constexpr size_t shm = 0;  // both tile buffers are static LDS
   const uint32_t qpb = kLowpWaves * QF * 16;
   // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid.
@@ -272,6 +319,7 @@ hipError_t launch_scan_bf16(const LowpScanArgs& a, int metric, uint32_t nslices,
     case kL2: return scan_bf16_m<kL2>(a, nslices, minonly, s);
     case kCos: return scan_bf16_m<kCos>(a, nslices, minonly, s);
     case kDot: return scan_bf16_m<kDot>(a, nslices, minonly, s);
+    case kMixed: return scan_bf16_m<kMixed>(a, nslices, minonly, s);
   }
   return hipErrorInvalidValue;
 }

@@ -197,6 +197,7 @@ __global__ void k_query_prep8(const double* __restrict__ Q, size_t ldq, uint32_t
   if (i >= nq_pad) return;
   uint8_t* q8 = Q8 + (size_t)i * dp;
   QParams p{};
+  p.metric = metric;
   if (i >= nq) {
     for (size_t c = 0; c < dp; c++) q8[c] = 0;
     p.flags = 1;
@@ -367,9 +368,10 @@ __global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
         B[f][ks] = *reinterpret_cast<const i32x8*>(Q8 + (size_t)q * G::RB + rg * 32 + ks * 128);
       }
     }
-    P[f] = a.qp[min(q, a.nq - 1)].P;
+    const QParams& qq = a.qp[min(q, a.nq - 1)];
+    P[f] = qq.P;
     float t0 = -INFINITY;
-    if (q < a.nq) {
+    if (q < a.nq && !qq.flags) {
       t0 = INFINITY;
       if (a.tau0) {
         const float t = a.tau0[(size_t)q * a.tau0_stride];
