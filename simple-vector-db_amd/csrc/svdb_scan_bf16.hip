@@ -139,11 +139,18 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
     }
   }
 
-  // Pending subtile (MFMAs issued, epilogue not yet run).
+  // Pending subtile (MFMAs issued, epilogue not yet run).  It starts with
+  // values whose keys are all +inf (L2: w = +inf with P > 0; dot: w = -inf
+  // with P < 0; cosine: pen = +inf), so the epilogue runs unconditionally
+  // and shares its basic block with the next subtile's MFMAs.
   f32x4 pacc[QF];
-  float4 pw = make_float4(0.f, 0.f, 0.f, 0.f), ppen = pw;
+#pragma unroll
+  for (int f = 0; f < QF; f++) pacc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const float w_init = wcos ? 0.f : (wm == kDot ? -INFINITY : INFINITY);
+  float4 pw = make_float4(w_init, w_init, w_init, w_init);
+  const float p_init = wcos ? INFINITY : 0.f;
+  float4 ppen = make_float4(p_init, p_init, p_init, p_init);
   uint32_t prow = 0;
-  bool pending = false;
 
   auto key = [&](const f32x4 (&acc)[QF], int f, int r, const float (&wv)[4],
                  const float (&pv)[4]) {
@@ -153,9 +160,18 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
   // One test per subtile for all fragments: in the common case no key
   // beats its list's 8th entry and the insertion code is skipped with a
   // single branch; the rare insertion path recomputes the keys.
-  auto epilogue = [&](const f32x4 (&acc)[QF], const float4& w, const float4& pen, uint32_t srow) {
-    const float wv[4] = {w.x, w.y, w.z, w.w};
-    const float pv[4] = {pen.x, pen.y, pen.z, pen.w};
+  auto need_of = [&](const f32x4 (&acc)[QF], const float (&wv)[4], const float (&pv)[4]) {
+    bool need = false;
+#pragma unroll
+    for (int f = 0; f < QF; f++) {
+      const float mn = fminf(fminf(key(acc, f, 0, wv, pv), key(acc, f, 1, wv, pv)),
+                             fminf(key(acc, f, 2, wv, pv), key(acc, f, 3, wv, pv)));
+      need |= mn < tau[f];
+    }
+    return need;
+  };
+  auto insert_all = [&](const f32x4 (&acc)[QF], const float (&wv)[4], const float (&pv)[4],
+                        uint32_t srow) {
     if constexpr (MINONLY) {
 #pragma unroll
       for (int f = 0; f < QF; f++) {
@@ -164,16 +180,6 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
         tau[f] = fminf(tau[f], mn);
       }
       return;
-    }
-    if constexpr (!MIXED) {  // (the mixed scan has no registers to spare for it)
-      bool need = false;
-#pragma unroll
-      for (int f = 0; f < QF; f++) {
-        const float mn = fminf(fminf(key(acc, f, 0, wv, pv), key(acc, f, 1, wv, pv)),
-                               fminf(key(acc, f, 2, wv, pv), key(acc, f, 3, wv, pv)));
-        need |= mn < tau[f];
-      }
-      if (!need) return;
     }
 #pragma unroll
     for (int f = 0; f < QF; f++) {
@@ -202,6 +208,14 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
         tau[f] = lk[f][LP - 1];
       }
     }
+  };
+  auto epilogue = [&](const f32x4 (&acc)[QF], const float4& w, const float4& pen, uint32_t srow) {
+    const float wv[4] = {w.x, w.y, w.z, w.w};
+    const float pv[4] = {pen.x, pen.y, pen.z, pen.w};
+    // (the mixed scan has no registers to spare for the combined test)
+    if constexpr (!MINONLY && !MIXED)
+      if (!need_of(acc, wv, pv)) return;
+    insert_all(acc, wv, pv, srow);
   };
 
   // Two LDS buffers as distinct objects, picked at compile time (tiles are
@@ -241,19 +255,32 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
 #pragma unroll
           for (int f = 0; f < QF; f++)
             acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
-        if (pending) epilogue(pacc, pw, ppen, prow);
+        // the previous subtile's epilogue: its common-case test is VALU work
+        // in the same basic block as the MFMAs above, interleaved with them
+        // (one MFMA, then two VALU instructions) so it runs in their shadow
+        {
+          const float wv[4] = {pw.x, pw.y, pw.z, pw.w};
+          const float pv[4] = {ppen.x, ppen.y, ppen.z, ppen.w};
+          bool need = true;
+          if constexpr (!MINONLY && !MIXED) need = need_of(pacc, wv, pv);
+#pragma unroll
+          for (int i = 0; i < KS * QF; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+          }
+          if (need) insert_all(pacc, wv, pv, prow);
+        }
 #pragma unroll
         for (int f = 0; f < QF; f++) pacc[f] = acc[f];
         pw = w;
         ppen = pen;
         prow = trow + rl;
-        pending = true;
       }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
     }
   }
-  if (pending) epilogue(pacc, pw, ppen, prow);
+  epilogue(pacc, pw, ppen, prow);
 
   const uint32_t list = slice * kLowpLanesPerQuery + rg;
 #pragma unroll

@@ -90,16 +90,41 @@ def test_bf16_gaussian(gauss_lowp=None):
     db.close()
 
 
+def _check_mixed(X, Q, ks, ms, ids, sc):
+    for m in ("l2", "cos", "dot"):
+        for k in sorted(set(int(v) for v in ks)):
+            sel = np.flatnonzero((np.asarray(ms) == m) & (np.asarray(ks) == k))
+            if sel.size == 0:
+                continue
+            rid, rsc = O.topk(X, Q[sel], k, m)
+            assert_same(ids[sel, :k], sc[sel, :k], rid, rsc, f"mixed {m} k={k}")
+
+
 def test_auto_mixed_batch(mix):
+    """Several metrics in one call: one mixed-metric first pass, every query
+    checked against the oracle."""
     X, Q, db = mix
     rng = np.random.default_rng(3)
     ks = rng.choice([1, 10, 100], size=Q.shape[0])
     ms = rng.choice(["l2", "cos", "dot"], size=Q.shape[0])
     say("auto precision, mixed k/metric")
+    s0 = db.stats()
     ids, sc = db.search(Q, ks, list(ms), "auto")
-    for i in range(0, Q.shape[0], 23):
-        rid, rsc = O.topk(X, Q[i:i + 1], int(ks[i]), str(ms[i]))
-        assert_same(ids[i:i + 1, :ks[i]], sc[i:i + 1, :ks[i]], rid, rsc, f"query {i}")
+    s1 = db.stats()
+    _check_mixed(X, Q, ks, ms, ids, sc)
+    assert s1["certified"] - s0["certified"] >= 0.9 * Q.shape[0]
+
+
+def test_mixed_unbalanced_ranges(mix):
+    """Metric ranges of 1, 130 and 65 queries (padding between wavefront-
+    aligned ranges) with a degenerate query inside one range."""
+    X, Q, db = mix
+    Qm = np.vstack([Q[:196], np.zeros((1, Q.shape[1]))])
+    ms = ["dot"] + ["l2"] * 130 + ["cos"] * 65 + ["l2"]
+    ks = np.array([7] * 131 + [3] * 65 + [7])
+    say("mixed ranges 1/131/65 with a zero query")
+    ids, sc = db.search(Qm, ks, ms, "auto")
+    _check_mixed(X, Qm, ks, ms, ids, sc)
 
 
 def test_bf16_after_mutations_and_degenerate_rows():
