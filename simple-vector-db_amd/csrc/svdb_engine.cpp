@@ -653,7 +653,7 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
 // gathered and answered by the FP64 scan.
 int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k,
                      uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
-  LowpRange r{metric, 0, nq, 0};
+  LowpRange r{metric, 0, nq, k, 0};
   return run_lowp_ranges(dQ, ldq, &r, 1, k, dids, dscores, ostride, fp8);
 }
 
@@ -662,12 +662,15 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
 // several ranges (BF16 rows <= 256 wide only) one mixed-metric first pass
 // serves them all: each range takes a run of slots starting on a wavefront
 // boundary (padding slots are flagged and never scanned); the pre-pass,
-// rerank, widening and fallback then run per range.
+// shortlist, rerank, widening and fallback then run per range, each with
+// its own k and shortlist width (k is the largest over ranges).
 int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, int k,
                             uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
   int rc;
-  const bool mixed = nr > 1;
-  if (mixed && (fp8 || sh_.dp > 256)) {
+  const bool mixed = nr > 1;  // several ranges: slots aligned per range
+  bool multi_metric = false;  // ... of different metrics: the mixed-metric scan
+  for (int i = 1; i < nr; i++) multi_metric |= R[i].metric != R[0].metric;
+  if (multi_metric && (fp8 || sh_.dp > 256)) {
     set_error(SVDB_EINVAL, "mixed-metric first pass needs the BF16 shadow of rows <= 256 wide");
     return SVDB_EINVAL;
   }
@@ -675,6 +678,7 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   for (int i = 0; i < nr; i++) {
     R[i].soff = nq;
     nq += mixed ? (uint32_t)round_up(R[i].n, kSlotAlign) : R[i].n;
+    if (R[i].k <= 0 || R[i].k > k) R[i].k = k;
   }
   const size_t dp = fp8 ? sh8_.dp : sh_.dp;
   const size_t qbytes = fp8 ? 1 : sizeof(uint16_t);  // bytes per query element
@@ -684,8 +688,12 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   // 256 once k passes 32, where the k-th and L-th neighbours need more room
   // than the first-pass error bound leaves in a 2k-wide list.  FP8 starts at
   // the configured width (or k) and relies on the wider retries below.
-  const uint32_t L = (uint32_t)std::min<size_t>(
-      kMaxShort, std::max<size_t>({shortlist_, (size_t)k, (!fp8 && k > 32) ? (size_t)256 : 0}));
+  auto width = [&](int kk) {
+    return (uint32_t)std::min<size_t>(
+        kMaxShort,
+        std::max<size_t>({shortlist_, (size_t)kk, (!fp8 && kk > 32) ? (size_t)256 : 0}));
+  };
+  const uint32_t L = width(k);  // the widest range's
   const uint32_t nrows_pad = (uint32_t)round_up(nphys_, kLowpTile);
   // Row slices: at least 8 (each query then has >= 32 candidate lists, so
   // the lists' own bound sits well past the shortlist's), more when there
@@ -759,10 +767,11 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     return fp8 ? launch_scan_fp8(x, m, nsl, minonly, stream_)
                : launch_scan_bf16(x, m, nsl, minonly, stream_);
   };
-  const int smetric = mixed ? kMixed : R[0].metric;
+  const int smetric = multi_metric ? kMixed : R[0].metric;
   LowpScanArgs sa{fp8 ? (const uint16_t*)sh8_.X8 : sh_.Xb, dp,
-                  mixed ? sh_.nx : w0_of(R[0].metric), mixed ? sh_.pen : w1_of(R[0].metric),
-                  mixed ? sh_.xn : nullptr, mixed ? sh_.inv : nullptr,
+                  multi_metric ? sh_.nx : w0_of(R[0].metric),
+                  multi_metric ? sh_.pen : w1_of(R[0].metric),
+                  multi_metric ? sh_.xn : nullptr, multi_metric ? sh_.inv : nullptr,
                   (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, rps, rps, nrows_pad, NL,
                   (float*)s_ck_, (uint32_t*)s_cs_, nullptr, 0};
   if (pre) {
@@ -780,11 +789,11 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
       pa.NL = NL_pre;
       pa.cand_key = (float*)s_ck_ + (size_t)R[i].soff * NL_pre;
       HIPCHK(scan(pa, R[i].metric, ns_pre, true), "first-pass scan (sample)");
+      // thresholds in s_beta_, read by the main scan before the shortlist
+      // below overwrites them (padding slots' values are never used)
+      HIPCHK(launch_kth_key((const float*)s_ck_ + (size_t)R[i].soff * NL_pre, R[i].n, NL_pre,
+                            width(R[i].k), (float*)s_beta_ + R[i].soff, stream_), "k_kth_key");
     }
-    // thresholds in s_beta_, read by the main scan before the shortlist
-    // below overwrites them (padding slots' values are never used)
-    HIPCHK(launch_kth_key((const float*)s_ck_, nq, NL_pre, L, (float*)s_beta_, stream_),
-           "k_kth_key");
     prof_end(SVDB_K_PREPASS);
     sa.tau0 = (const float*)s_beta_;
     sa.tau0_stride = 1;
@@ -792,17 +801,27 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   prof_begin(SVDB_K_SCAN_LOWP);
   HIPCHK(scan(sa, smetric, ns, false), "first-pass scan");
   prof_end(SVDB_K_SCAN_LOWP);
-  ShortlistArgs sl{(const float*)s_ck_, (const uint32_t*)s_cs_, nq, NL, L, (uint32_t*)s_ss_,
-                   (float*)s_sk_, (float*)s_beta_};
+  // range i's shortlist: width Li = width(k_i), at slot offset soff (row
+  // stride Li inside the range's block of soff*L .. )
   prof_begin(SVDB_K_SHORTLIST);
-  HIPCHK(launch_shortlist(sl, stream_), "k_shortlist");
+  for (int i = 0; i < nr; i++) {
+    const uint32_t Li = width(R[i].k);
+    ShortlistArgs sl{(const float*)s_ck_ + (size_t)R[i].soff * NL * kLowpLP,
+                     (const uint32_t*)s_cs_ + (size_t)R[i].soff * NL * kLowpLP, R[i].n, NL, Li,
+                     (uint32_t*)s_ss_ + (size_t)R[i].soff * L, (float*)s_sk_ + (size_t)R[i].soff * L,
+                     (float*)s_beta_ + R[i].soff};
+    HIPCHK(launch_shortlist(sl, stream_), "k_shortlist");
+  }
   prof_end(SVDB_K_SHORTLIST);
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
+  const uint32_t Lmax = L;  // shortlist block stride per slot
   double mx = 0.0;
   for (int ri = 0; ri < nr; ri++) {
     const LowpRange& rg = R[ri];
     const int metric = rg.metric;
     const uint32_t nqr = rg.n;
+    const int k = rg.k;                // this range's k (shadows the call's)
+    const uint32_t L = width(k);       // and its shortlist width
     if (nqr == 0) continue;
     // this range's views: slot-space arrays at soff, query-space at qoff
     const double* rQ = dQ + (size_t)rg.qoff * ldq;
@@ -814,8 +833,8 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     uint32_t* fail_list = fail_count + 1;  // may be redirected to a widening retry's list
     HIPCHK(hipMemsetAsync(fail_count, 0, sizeof(uint32_t), stream_), "memset");
     RerankArgs ra{sv, rQ, ldq, (uint32_t)d_, (const QParams*)s_qp_ + rg.soff,
-                  (const uint32_t*)s_ss_ + (size_t)rg.soff * L,
-                  (const float*)s_sk_ + (size_t)rg.soff * L, (const float*)s_beta_ + rg.soff,
+                  (const uint32_t*)s_ss_ + (size_t)rg.soff * Lmax,
+                  (const float*)s_sk_ + (size_t)rg.soff * Lmax, (const float*)s_beta_ + rg.soff,
                   degen, degen_count, w0_of(metric), w1_of(metric), nqr, L, (uint32_t)k,
                   ostride, rids, rsc, fail_count, fail_list, (double*)s_err_ + rg.soff};
     prof_begin(SVDB_K_RERANK);
@@ -1043,18 +1062,22 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   // Several metrics in one BF16 batch (rows <= 256 wide): one mixed-metric
   // first pass for all of them, queries grouped by metric.
   if (which == 1 && sh_.dp <= 256 && nq <= step) {
-    size_t cnt[3] = {0, 0, 0};
-    for (size_t i = 0; i < nq; i++) cnt[metric[i]]++;
-    if ((cnt[0] > 0) + (cnt[1] > 0) + (cnt[2] > 0) >= 2) {
+    {
+      // one range per metric, all at the batch's largest k (splitting by k
+      // as well costs more in latency-bound per-range shortlists than the
+      // narrower lists save)
       LowpRange R[3];
       int nr = 0;
       sel.clear();
       for (int m = 0; m < 3; m++) {
-        if (!cnt[m]) continue;
-        R[nr++] = LowpRange{m, (uint32_t)sel.size(), (uint32_t)cnt[m], 0};
+        const size_t first = sel.size();
         for (size_t i = 0; i < nq; i++)
           if (metric[i] == m) sel.push_back(i);
+        if (sel.size() > first)
+          R[nr++] = LowpRange{m, (uint32_t)first, (uint32_t)(sel.size() - first),
+                              (int)kmax_all, 0};
       }
+      if (nr < 2) goto per_metric;  // one metric: the plain path below
       const size_t n = sel.size();
       hq.assign(n * D, 0.0);
       for (size_t t = 0; t < n; t++) std::memcpy(&hq[t * D], Q + sel[t] * ldq, D * sizeof(double));
@@ -1083,6 +1106,7 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
       return SVDB_OK;
     }
   }
+per_metric:
   for (int m = 0; m < 3; m++) {
     sel.clear();
     for (size_t i = 0; i < nq; i++)

@@ -114,6 +114,7 @@ class Engine {
   struct LowpRange {
     int metric;
     uint32_t qoff, n;  // query rows qoff .. qoff+n-1
+    int k;             // neighbours wanted (0: the call's k)
     uint32_t soff;     // first slot (set by run_lowp_ranges)
   };
   int run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k, uint64_t* dids,

@@ -256,18 +256,14 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
           for (int f = 0; f < QF; f++)
             acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
         // the previous subtile's epilogue: its common-case test is VALU work
-        // in the same basic block as the MFMAs above, interleaved with them
-        // (one MFMA, then two VALU instructions) so it runs in their shadow
+        // in the same basic block as the MFMAs above, free to be scheduled
+        // among them
         {
           const float wv[4] = {pw.x, pw.y, pw.z, pw.w};
           const float pv[4] = {ppen.x, ppen.y, ppen.z, ppen.w};
           bool need = true;
           if constexpr (!MINONLY && !MIXED) need = need_of(pacc, wv, pv);
-#pragma unroll
-          for (int i = 0; i < KS * QF; i++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
-          }
+
           if (need) insert_all(pacc, wv, pv, prow);
         }
 #pragma unroll
