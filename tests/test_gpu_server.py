@@ -155,3 +155,69 @@ def test_admission_limits():
         assert info["tick_queries"] <= 64
     srv.stop()
     db.close()
+
+
+def test_concurrent_submitters():
+    """Four client threads submit at once (searches and mutations); every
+    result equals the same request run alone, replayed in ticket order (the
+    order the server admitted them)."""
+    import threading
+
+    rows = workloads.mixture(40_000, D, seed=43, n_centers=32)
+    pool = workloads.mixture(2048, D, seed=44, n_centers=32)
+    db = _store(rows)
+    srv = svdb.Server(db, max_batch=4096)
+    lock = threading.Lock()
+    done = []  # (ticket, plan entry, result)
+    size = [rows.shape[0]]
+
+    def client(seed):
+        rng = np.random.default_rng(seed)
+        mine = []
+        for _ in range(30):
+            if rng.random() < 0.08:
+                with lock:  # keep indices valid for whatever order the server sees
+                    kind = int(rng.integers(2))
+                    if kind == 0:
+                        p = ("insert", rng.standard_normal(D))
+                        t = srv.submit_insert(p[1])
+                        size[0] += 1
+                    else:
+                        p = ("update", int(rng.integers(size[0])), rng.standard_normal(D))
+                        t = srv.submit_update(p[1], p[2])
+            else:
+                nq = max(1, int(np.exp(rng.uniform(0, np.log(128)))))
+                o = int(rng.integers(0, pool.shape[0] - nq + 1))
+                p = ("search", pool[o:o + nq].copy(), int(rng.choice([1, 10, 100])),
+                     ["l2", "cos", "dot"][rng.integers(3)])
+                t = srv.submit_search(p[1], p[2], p[3])
+            mine.append((t, p))
+        for t, p in mine:
+            r = srv.wait(t)
+            with lock:
+                done.append((t, p, r))
+
+    th = [threading.Thread(target=client, args=(100 + i,)) for i in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    srv.stop()
+    db.close()
+    assert len(done) == 120
+    done.sort(key=lambda e: e[0])
+    db = _store(rows)
+    shared = 0
+    for t, p, (rc, ids, sc, info) in done:
+        assert rc == 0, p[0]
+        if p[0] == "search":
+            ids_a, sc_a = db.search(p[1], p[2], p[3], precision="auto")
+            assert np.array_equal(ids, ids_a), (t, p[1].shape, p[2], p[3])
+            assert np.array_equal(_bits(sc), _bits(sc_a)), (t, p[2], p[3])
+            shared += info["tick_queries"] > p[1].shape[0]
+        elif p[0] == "insert":
+            assert db.insert(p[1]) == ids
+        else:
+            db.update(p[1], p[2])
+    assert shared > 0
+    db.close()
