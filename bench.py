@@ -36,6 +36,113 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def fp64_c2(args):
+    """The plain FP64 GPU scan on its own workload: 1,000,000 x 128 rows
+    i.i.d. N(0,1), 10,000 queries in batches of 1,024, top-10 for each metric;
+    ids and scores checked against the CPU oracle on the first 64 queries."""
+    import svdb
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((1_000_000, 128))
+    Q = rng.standard_normal((10_240, 128))
+    db = svdb.Database(kd_dimension=3)
+    db.bulk_insert(X)
+    dQ = svdb.DeviceArray.from_host(Q)
+    B, k = 1024, 10
+    dids = svdb.DeviceArray((B, k), np.uint64)
+    dsc = svdb.DeviceArray((B, k), np.float64)
+    out = {"workload": "1000000x128 N(0,1), batches of 1024, top-10, plain FP64 GPU scan"}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    for metric in ("l2", "cos", "dot"):
+        db.search_device(dQ.ptr, B, 128, k, metric, "fp64", dids.ptr, dsc.ptr)  # warm
+        svdb.device_sync()
+        db.set_profiling(True)
+        t = time.perf_counter()
+        for b in range(10):
+            db.search_device(dQ.ptr + b * B * 128 * 8, B, 128, k, metric, "fp64", dids.ptr,
+                             dsc.ptr)
+        svdb.device_sync()
+        el = time.perf_counter() - t
+        kt = db.kernel_times()[svdb.K_SCAN_FP64]
+        ids, sc = dids.to_host(), dsc.to_host()  # the last batch
+        rid, rsc = oracle.topk(X, Q[9 * B:9 * B + 64], k, metric)
+        same = bool(np.array_equal(ids[:64], rid) and
+                    np.array_equal(sc[:64].view(np.uint64), rsc.view(np.uint64)))
+        scan_ms = kt[1] / max(1, kt[0])
+        tf = 2.0 * X.shape[0] * 128 * B / (scan_ms * 1e-3) / 1e12
+        out[metric] = {"qps": round(10 * B / el, 1), "scan_ms_per_batch": round(scan_ms, 3),
+                       "tflops": round(tf, 2), "frac_of_fp64_peak": round(tf / PEAK_FP64_TFLOPS, 4),
+                       "ids_scores_equal_cpu_oracle_64": same}
+        log(f"fp64 scan C2 {metric}: {out[metric]}")
+    for a in (dQ, dids, dsc):
+        a.free()
+    db.close()
+    return out
+
+
+def reference_c1():
+    """The reference's own CPU find-nearest (its kdtree.c, built from its
+    sources into oracle/_ref) on the CPU-runnable workload: 10,000 x 128
+    uniform[-1,1] rows (seed 0), 256 queries, kd-tree dimension 3; beside it
+    the same answers from this engine through kdtree_nearest, one call per
+    query.  None when the reference build is absent."""
+    import ctypes as C
+    import svdb
+    ref = os.path.join(ROOT, "oracle", "_ref", "libsvdbref.so")
+    if not os.path.exists(ref):
+        return None
+    lib = C.CDLL(ref)
+    vp = C.c_void_p
+    lib.kdtree_create.restype = vp
+    lib.kdtree_create.argtypes = [C.c_size_t]
+    lib.kdtree_insert.argtypes = [vp, C.POINTER(C.c_double), C.c_size_t]
+    lib.kdtree_nearest.restype = C.c_size_t
+    lib.kdtree_nearest.argtypes = [vp, C.POINTER(C.c_double)]
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-1, 1, (10000, 128))
+    Q = rng.uniform(-1, 1, (256, 128))
+    # the reference prints on every kd-tree insert: silence fd 1 meanwhile
+    libc = C.CDLL(None)
+    sys.stdout.flush()
+    libc.fflush(None)
+    saved, null = os.dup(1), os.open(os.devnull, os.O_WRONLY)
+    os.dup2(null, 1)
+    try:
+        tree = lib.kdtree_create(3)
+        keep = [np.ascontiguousarray(X[i]) for i in range(X.shape[0])]
+        for i, r in enumerate(keep):
+            lib.kdtree_insert(tree, r.ctypes.data_as(C.POINTER(C.c_double)), i)
+        libc.fflush(None)
+    finally:
+        os.dup2(saved, 1)
+        os.close(null)
+        os.close(saved)
+    qs = [np.ascontiguousarray(q) for q in Q]
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        ref_ids = [lib.kdtree_nearest(tree, q.ctypes.data_as(C.POINTER(C.c_double))) for q in qs]
+    ref_qps = reps * len(qs) / (time.perf_counter() - t)
+    db = svdb.Database(kd_dimension=3)
+    db.bulk_insert(X)
+    ours = [db.kd_nearest(q) for q in qs]  # warm
+    t = time.perf_counter()
+    ours = [db.kd_nearest(q) for q in qs]
+    one_qps = len(qs) / (time.perf_counter() - t)
+    ids, _ = db.search(Q[:, :3], 1, "l2", "fp64")  # batched: the first 3 coordinates
+    t = time.perf_counter()
+    for _ in range(reps):
+        ids, _ = db.search(Q[:, :3], 1, "l2", "fp64")
+    batch_qps = reps * len(qs) / (time.perf_counter() - t)
+    db.close()
+    return {"workload": "10000x128 uniform[-1,1] (seed 0), 256 queries, find-nearest over the "
+                        "first 3 coordinates (kd-tree dimension 3)",
+            "reference_cpu_kdtree_qps": round(ref_qps, 1), "cores": 1,
+            "gpu_kdtree_nearest_qps": round(one_qps, 1),
+            "gpu_batched_256_qps": round(batch_qps, 1),
+            "ids_equal": bool(ref_ids == [int(v) for v in ours] == [int(v) for v in ids[:, 0]])}
+
+
 def serving_plan(rng, n_req, pool_rows, store_rows, mut_frac=0.05):
     """Requests of the serving workload: batch sizes log-uniform in 1..512,
     k in {1, 10, 100}, metric uniform, mut_frac of them mutations (insert,
@@ -435,6 +542,13 @@ def main():
                          f"AVX2, {threads} threads), {el:.1f}s"}
         log(f"cpu oracle: {n / el:.2f} q/s on {threads} threads; ids+scores identical: {oracle_exact}")
 
+    # ---------------- the plain FP64 scan on its own workload; the reference
+    # ---------------- CPU path on its CPU-runnable workload
+    c2 = fp64_c2(args)
+    c1 = reference_c1() if not args.no_cpu else None
+    if c1:
+        log(f"reference C1: {c1}")
+
     # ---------------- serving loop: Poisson arrivals, mixed requests -------
     serving = None
     if not args.no_serving:
@@ -501,6 +615,8 @@ def main():
                           "speedup_of_bf16_path": round(qps / fp64_qps, 2)},
         "cpu_baseline": cpu,
         "serving": serving,
+        "fp64_gpu_scan_c2": c2,
+        "reference_cpu_c1": c1,
     }
     print(json.dumps(out), flush=True)
     db.close()

@@ -65,8 +65,9 @@ enum {
 /* Top-k for nq host queries (row i at queries + i*dim).  k[i] in
  * [1, SVDB_MAX_K] and metric[i] per query; out_stride >= max k[i].
  * ids_out[i*out_stride + j], scores_out[...]: j-th best; slots past the
- * number of stored rows get (size_t)-1 and NaN.  dim must equal the store's
- * vector dimension. */
+ * number of stored rows get (size_t)-1 and NaN.  dim is the store's vector
+ * dimension, or, for Euclidean only, less: the first dim coordinates are
+ * compared, as the reference's kd-tree does (kdtree.c:134-137). */
 int svdb_search(VectorDatabase* db, const double* queries, size_t nq, size_t dim,
                 const size_t* k, const int* metric, int precision, size_t out_stride,
                 size_t* ids_out, double* scores_out);

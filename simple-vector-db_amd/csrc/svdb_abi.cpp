@@ -510,8 +510,10 @@ int svdb_search(VectorDatabase* db, const double* queries, size_t nq, size_t dim
       svdb::set_error(SVDB_EINVAL, "unknown metric");
       return SVDB_EINVAL;
     }
-  if (e->dim() != 0 && dim != e->dim()) {
-    svdb::set_error(SVDB_EINVAL, "query dimension differs from the store's");
+  // dim below the store's: Euclidean over the first dim coordinates (the
+  // kd-tree's projection); the engine refuses it for cosine and dot
+  if (e->dim() != 0 && (dim == 0 || dim > e->dim())) {
+    svdb::set_error(SVDB_EINVAL, "query dimension exceeds the store's");
     return SVDB_EINVAL;
   }
   return e->search_host(queries, nq, dim, dim, k, metric, precision, out_stride, ids_out, scores_out);

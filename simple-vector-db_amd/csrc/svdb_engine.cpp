@@ -166,6 +166,31 @@ int Engine::scratch(void** p, size_t* have, size_t need) {
   return SVDB_OK;
 }
 
+// Pinned host staging shared by row uploads and searches (both run under
+// mu_ and synchronise the stream before returning, so the buffer is free on
+// entry).  Searches stage their query rows, k values, ids and scores here:
+// copies from pinned memory are plain DMAs, where pageable ones go through
+// the runtime's bounce buffers and cost tens of microseconds each, most of a
+// one-query call.  Requests above kMaxStage return nullptr and the caller
+// uses pageable vectors instead.
+static constexpr size_t kMaxStage = 256ull << 20;
+
+void* Engine::host_stage(size_t bytes) {
+  if (bytes > kMaxStage) return nullptr;
+  if (h_stage_ && h_stage_n_ >= bytes) return h_stage_;
+  if (h_stage_) (void)hipHostFree(h_stage_);
+  h_stage_ = nullptr;
+  h_stage_n_ = 0;
+  const size_t n = round_up(std::max<size_t>(bytes, 1u << 20), 1u << 20);
+  if (hipHostMalloc(&h_stage_, n) != hipSuccess) {
+    h_stage_ = nullptr;
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  h_stage_n_ = n;
+  return h_stage_;
+}
+
 int Engine::ensure_dim(size_t d) {
   if (d == 0) {
     set_error(SVDB_EINVAL, "vector dimension 0");
@@ -449,15 +474,13 @@ int Engine::append(const double* rows, size_t n, size_t lds, size_t dim, uint32_
     // the device layout (stride ld, zero padding), one DMA per chunk.
     const size_t row_b = ld_ * sizeof(double);
     const size_t chunk = std::max<size_t>(1, std::min<size_t>(n, (64u << 20) / row_b));
-    if (!h_stage_ || h_stage_n_ < chunk * row_b) {
-      if (h_stage_) (void)hipHostFree(h_stage_);
-      h_stage_ = nullptr;
-      HIPCHK(hipHostMalloc(&h_stage_, chunk * row_b), "hipHostMalloc");
-      h_stage_n_ = chunk * row_b;
+    double* st = static_cast<double*>(host_stage(chunk * row_b));
+    if (!st) {
+      set_error(SVDB_EDEVICE, "hipHostMalloc failed");
+      return SVDB_EDEVICE;
     }
     for (size_t r0 = 0; r0 < n; r0 += chunk) {
       const size_t m = std::min(chunk, n - r0);
-      double* st = static_cast<double*>(h_stage_);
       for (size_t r = 0; r < m; r++) {
         std::memcpy(st + r * ld_, rows + (r0 + r) * lds, d_ * sizeof(double));
         if (ld_ != d_) std::memset(st + r * ld_ + d_, 0, (ld_ - d_) * sizeof(double));
@@ -1054,11 +1077,35 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   const bool lowp = which != 0;
   const size_t step = lowp ? kMaxBatchLowp : kMaxBatch;
 
-  std::vector<double> hq;
+  std::vector<size_t> sel;
+  // Host side of one chunk (query rows, k values, ids, scores): pinned
+  // staging when it fits under its cap, else pageable vectors.
+  std::vector<double> hq, hsc;
   std::vector<int> hk;
   std::vector<uint64_t> hid;
-  std::vector<double> hsc;
-  std::vector<size_t> sel;
+  double* pq = nullptr;
+  int* pk = nullptr;
+  uint64_t* pid = nullptr;
+  double* psc = nullptr;
+  auto host_bufs = [&](size_t n, size_t kk) {
+    const size_t qb = round_up(n * D * sizeof(double), 64), kb = round_up(n * sizeof(int), 64),
+                 ob = round_up(n * kk * sizeof(uint64_t), 64);
+    if (char* st = static_cast<char*>(host_stage(qb + kb + 2 * ob))) {
+      pq = reinterpret_cast<double*>(st);
+      pk = reinterpret_cast<int*>(st + qb);
+      pid = reinterpret_cast<uint64_t*>(st + qb + kb);
+      psc = reinterpret_cast<double*>(st + qb + kb + ob);
+    } else {
+      hq.resize(n * D);
+      hk.resize(n);
+      hid.resize(n * kk);
+      hsc.resize(n * kk);
+      pq = hq.data();
+      pk = hk.data();
+      pid = hid.data();
+      psc = hsc.data();
+    }
+  };
   // Several metrics in one BF16 batch (rows <= 256 wide): one mixed-metric
   // first pass for all of them, queries grouped by metric.
   if (which == 1 && sh_.dp <= 256 && nq <= step) {
@@ -1079,26 +1126,24 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
       }
       if (nr < 2) goto per_metric;  // one metric: the plain path below
       const size_t n = sel.size();
-      hq.assign(n * D, 0.0);
-      for (size_t t = 0; t < n; t++) std::memcpy(&hq[t * D], Q + sel[t] * ldq, D * sizeof(double));
       if ((rc = scratch(&s_q_, &s_q_n_, n * D * sizeof(double)))) return rc;
       if ((rc = scratch(&s_ids_, &s_ids_n_, n * kmax_all * sizeof(uint64_t)))) return rc;
       if ((rc = scratch(&s_sc_, &s_sc_n_, n * kmax_all * sizeof(double)))) return rc;
-      HIPCHK(hipMemcpyAsync(s_q_, hq.data(), n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
+      host_bufs(n, kmax_all);
+      for (size_t t = 0; t < n; t++) std::memcpy(pq + t * D, Q + sel[t] * ldq, D * sizeof(double));
+      HIPCHK(hipMemcpyAsync(s_q_, pq, n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
       rc = run_lowp_ranges((const double*)s_q_, D, R, nr, (int)kmax_all, (uint64_t*)s_ids_,
                            (double*)s_sc_, (uint32_t)kmax_all, false);
       if (rc) return rc;
-      hid.resize(n * kmax_all);
-      hsc.resize(n * kmax_all);
-      HIPCHK(hipMemcpyAsync(hid.data(), s_ids_, n * kmax_all * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
-      HIPCHK(hipMemcpyAsync(hsc.data(), s_sc_, n * kmax_all * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
+      HIPCHK(hipMemcpyAsync(pid, s_ids_, n * kmax_all * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
+      HIPCHK(hipMemcpyAsync(psc, s_sc_, n * kmax_all * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
       HIPCHK(hipStreamSynchronize(stream_), "search");
       prof_collect();
       for (size_t t = 0; t < n; t++) {
         const size_t i = sel[t];
         for (size_t j = 0; j < k[i]; j++) {
-          ids[i * ostride + j] = (size_t)hid[t * kmax_all + j];
-          scores[i * ostride + j] = hsc[t * kmax_all + j];
+          ids[i * ostride + j] = (size_t)pid[t * kmax_all + j];
+          scores[i * ostride + j] = psc[t * kmax_all + j];
         }
       }
       st_.searches++;
@@ -1114,36 +1159,33 @@ per_metric:
     for (size_t b0 = 0; b0 < sel.size(); b0 += step) {
       const size_t n = std::min(step, sel.size() - b0);
       size_t kmax = 0;
-      hq.assign(n * D, 0.0);
-      hk.resize(n);
-      for (size_t t = 0; t < n; t++) {
-        const size_t i = sel[b0 + t];
-        std::memcpy(&hq[t * D], Q + i * ldq, D * sizeof(double));
-        hk[t] = (int)k[i];
-        kmax = std::max(kmax, k[i]);
-      }
+      for (size_t t = 0; t < n; t++) kmax = std::max(kmax, k[sel[b0 + t]]);
       if ((rc = scratch(&s_q_, &s_q_n_, n * D * sizeof(double)))) return rc;
       if ((rc = scratch(&s_k_, &s_k_n_, n * sizeof(int)))) return rc;
       if ((rc = scratch(&s_ids_, &s_ids_n_, n * kmax * sizeof(uint64_t)))) return rc;
       if ((rc = scratch(&s_sc_, &s_sc_n_, n * kmax * sizeof(double)))) return rc;
-      HIPCHK(hipMemcpyAsync(s_q_, hq.data(), n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
-      HIPCHK(hipMemcpyAsync(s_k_, hk.data(), n * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
+      host_bufs(n, kmax);
+      for (size_t t = 0; t < n; t++) {
+        const size_t i = sel[b0 + t];
+        std::memcpy(pq + t * D, Q + i * ldq, D * sizeof(double));
+        pk[t] = (int)k[i];
+      }
+      HIPCHK(hipMemcpyAsync(s_q_, pq, n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
+      HIPCHK(hipMemcpyAsync(s_k_, pk, n * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
       rc = lowp ? run_lowp((const double*)s_q_, D, (uint32_t)n, m, (int)kmax, (uint64_t*)s_ids_,
                            (double*)s_sc_, (uint32_t)kmax, which == 2)
                 : run_exact((const double*)s_q_, D, (uint32_t)n, (uint32_t)D, (const int*)s_k_, m,
                             (int)kmax, (uint64_t*)s_ids_, (double*)s_sc_, (uint32_t)kmax);
       if (rc) return rc;
-      hid.resize(n * kmax);
-      hsc.resize(n * kmax);
-      HIPCHK(hipMemcpyAsync(hid.data(), s_ids_, n * kmax * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
-      HIPCHK(hipMemcpyAsync(hsc.data(), s_sc_, n * kmax * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
+      HIPCHK(hipMemcpyAsync(pid, s_ids_, n * kmax * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
+      HIPCHK(hipMemcpyAsync(psc, s_sc_, n * kmax * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
       HIPCHK(hipStreamSynchronize(stream_), "search");
       prof_collect();
       for (size_t t = 0; t < n; t++) {
         const size_t i = sel[b0 + t];
         for (size_t j = 0; j < k[i]; j++) {
-          ids[i * ostride + j] = (size_t)hid[t * kmax + j];
-          scores[i * ostride + j] = hsc[t * kmax + j];
+          ids[i * ostride + j] = (size_t)pid[t * kmax + j];
+          scores[i * ostride + j] = psc[t * kmax + j];
         }
       }
     }

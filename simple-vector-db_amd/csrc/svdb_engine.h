@@ -109,6 +109,7 @@ class Engine {
   int dev_alloc(void** p, size_t bytes);
   void dev_free(void* p, size_t bytes);
   int scratch(void** p, size_t* have, size_t need);
+  void* host_stage(size_t bytes);  // pinned staging, grow-only; nullptr past its cap
   int run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk, int metric,
                 int kmax, uint64_t* dids, double* dscores, uint32_t ostride);
   struct LowpRange {
@@ -195,7 +196,7 @@ class Engine {
   void* s_cid_ = nullptr;    size_t s_cid_n_ = 0;   //   their ids
   void* s_csc_ = nullptr;    size_t s_csc_n_ = 0;   //   and scores
   uint32_t* h_fail_ = nullptr;  // pinned: fail count
-  void* h_stage_ = nullptr;     // pinned: row upload staging
+  void* h_stage_ = nullptr;     // pinned: row upload and search staging
   size_t h_stage_n_ = 0;
 
   struct ProfSlot {
