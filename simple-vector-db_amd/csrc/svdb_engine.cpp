@@ -123,6 +123,7 @@ Engine::~Engine() {
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h_fail_) (void)hipHostFree(h_fail_);
+  if (h_io_) (void)hipHostFree(h_io_);
   if (h_stage_) (void)hipHostFree(h_stage_);
   for (auto& e : ev_pool_) (void)hipEventDestroy(e);
   for (auto& ps : prof_pending_) {
@@ -676,7 +677,7 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
 // gathered and answered by the FP64 scan.
 int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k,
                      uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
-  LowpRange r{metric, 0, nq, k, 0};
+  LowpRange r{metric, 0, nq, k, 0, 0, 0, 0, 0};
   return run_lowp_ranges(dQ, ldq, &r, 1, k, dids, dscores, ostride, fp8);
 }
 
@@ -695,6 +696,12 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   for (int i = 1; i < nr; i++) multi_metric |= R[i].metric != R[0].metric;
   if (multi_metric && (fp8 || sh_.dp > 256)) {
     set_error(SVDB_EINVAL, "mixed-metric first pass needs the BF16 shadow of rows <= 256 wide");
+    return SVDB_EINVAL;
+  }
+  // (the FP8 pass hands leftovers to a nested BF16 pass, which reuses the
+  // candidate and fail-list buffers: one range only)
+  if (fp8 && nr > 1) {
+    set_error(SVDB_EINVAL, "FP8 first pass takes one query range");
     return SVDB_EINVAL;
   }
   uint32_t nq = 0;  // slots
@@ -718,25 +725,66 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   };
   const uint32_t L = width(k);  // the widest range's
   const uint32_t nrows_pad = (uint32_t)round_up(nphys_, kLowpTile);
-  // Row slices: at least 8 (each query then has >= 32 candidate lists, so
-  // the lists' own bound sits well past the shortlist's), more when there
-  // are few query blocks, each at least 4 tiles long.
   const uint32_t groups = nq_pad / qpb;
   const uint32_t tiles = nrows_pad / kLowpTile;
-  uint32_t ns = std::max<uint32_t>(8, (512 + groups - 1) / groups);
-  // Past k = 32 the lists' own bound (the smallest 8th entry) must stay
-  // well beyond the k-th neighbour: at least 512 lists per query.
-  if (k > 32) ns = std::max<uint32_t>(ns, 128);
-  // FP8: the widening retries take up to kMaxShort candidates from these
-  // lists; 2048 lists per query keep their own bound (the smallest 8th
-  // entry) clear of the 4096th candidate.
-  if (fp8) ns = std::max<uint32_t>(ns, 512);
-  ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, tiles / 4)));
-  const uint32_t rps = (uint32_t)round_up((tiles + ns - 1) / ns, 1) * kLowpTile;
-  ns = (nrows_pad + rps - 1) / rps;
-  // one set of slices per XCD: a multiple of 8 (trailing slices may be empty)
-  ns = (uint32_t)round_up(ns, 8);
-  const uint32_t NL = ns * kLowpLanesPerQuery;
+  // Row slices for a scan of grp query blocks wanting kk neighbours: at
+  // least 8 (each query then has >= 32 candidate lists, so the lists' own
+  // bound sits well past the shortlist's), more when there are few query
+  // blocks, each at least 4 tiles long.  One block per CU fits at a time,
+  // so the grid is kept at or just under two rounds of the 256 CUs (a few
+  // blocks past a round cost a whole round).
+  auto slicing = [&](uint32_t grp, int kk, LowpRange& r) {
+    uint32_t ns = std::max<uint32_t>(8, (512 / grp) & ~7u);
+    // Past k = 32 the lists' own bound (the smallest 8th entry) must stay
+    // well beyond the k-th neighbour: at least 512 lists per query.
+    if (kk > 32) ns = std::max<uint32_t>(ns, 128);
+    // FP8: the widening retries take up to kMaxShort candidates from these
+    // lists; 2048 lists per query keep their own bound (the smallest 8th
+    // entry) clear of the 4096th candidate.
+    if (fp8) ns = std::max<uint32_t>(ns, 512);
+    ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, tiles / 4)));
+    r.rps = (uint32_t)((tiles + ns - 1) / ns) * kLowpTile;
+    ns = (nrows_pad + r.rps - 1) / r.rps;
+    // one set of slices per XCD: a multiple of 8 (trailing slices may be empty)
+    r.ns = (uint32_t)round_up(ns, 8);
+    r.NL = r.ns * kLowpLanesPerQuery;
+  };
+  // Scan launches: consecutive ranges of one metric share one (their slots
+  // are contiguous; the padding slots between them are flagged), sliced for
+  // its own query count.  When the metric ranges fit one block of queries,
+  // one launch of the mixed-metric kernel serves them all instead (one pass
+  // over the rows rather than one per metric); past that, the single-metric
+  // kernels win: they skip most subtiles with one test, which the mixed
+  // kernel has no registers for.
+  const bool mixed_scan = multi_metric && groups <= 1;
+  struct ScanGroup {
+    int r0, r1, metric;
+    uint32_t soff, n;  // slots soff .. soff+n-1
+  };
+  std::vector<ScanGroup> sgs;
+  for (int i = 0; i < nr; i++) {
+    if (!sgs.empty() && (mixed_scan || R[i].metric == sgs.back().metric))
+      sgs.back().r1 = i + 1;
+    else
+      sgs.push_back({i, i + 1, mixed_scan ? kMixed : R[i].metric, R[i].soff, 0});
+  }
+  size_t cand_end = 0;  // candidate-list entries, all groups
+  for (auto& g : sgs) {
+    const LowpRange& last = R[g.r1 - 1];
+    g.n = last.soff + last.n - g.soff;
+    const uint32_t send = g.r1 < nr ? R[g.r1].soff : nq;
+    int kg = 0;
+    for (int i = g.r0; i < g.r1; i++) kg = std::max(kg, R[i].k);
+    LowpRange sl{};
+    slicing((g.n + qpb - 1) / qpb, kg, sl);
+    for (int i = g.r0; i < g.r1; i++) {
+      R[i].ns = sl.ns;
+      R[i].rps = sl.rps;
+      R[i].NL = sl.NL;
+      R[i].coff = cand_end + (size_t)(R[i].soff - g.soff) * sl.NL * kLowpLP;
+    }
+    cand_end += (size_t)(send - g.soff) * sl.NL * kLowpLP;
+  }
   // Sample pre-pass: the same scan over ns_pre evenly spaced runs of tp
   // tiles (at most 1/16 of the store), each lane keeping only the smallest
   // key of its rows.  The lanes' rows are disjoint, so the L-th smallest of
@@ -749,17 +797,30 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   const bool pre = prepass_ && tiles >= 16 * ns_pre && ns_pre * kLowpLanesPerQuery >= L;
   const uint32_t tp = pre ? std::min<uint32_t>(kPrepassMaxTiles, tiles / (16 * ns_pre)) : 0;
   const uint32_t NL_pre = ns_pre * kLowpLanesPerQuery;
-  const size_t ncand = (size_t)nq * std::max(NL * kLowpLP, pre ? NL_pre : 0u);
-  if ((rc = scratch(&s_qb_, &s_qb_n_, (size_t)nq_pad * dp * qbytes))) return rc;
+  const size_t ncand = std::max(cand_end, pre ? (size_t)nq * NL_pre : (size_t)0);
+  // query rows: nq_pad slots plus one block, since a launch over one range
+  // (at slot offset soff) reads whole blocks of rows from soff on
+  if ((rc = scratch(&s_qb_, &s_qb_n_, ((size_t)nq_pad + qpb) * dp * qbytes))) return rc;
   if ((rc = scratch(&s_qp_, &s_qp_n_, (size_t)nq_pad * sizeof(QParams)))) return rc;
   if ((rc = scratch(&s_ck_, &s_ck_n_, ncand * sizeof(float)))) return rc;
   if ((rc = scratch(&s_cs_, &s_cs_n_, ncand * sizeof(uint32_t)))) return rc;
   if ((rc = scratch(&s_sk_, &s_sk_n_, (size_t)nq * L * sizeof(float)))) return rc;
   if ((rc = scratch(&s_ss_, &s_ss_n_, (size_t)nq * L * sizeof(uint32_t)))) return rc;
   if ((rc = scratch(&s_beta_, &s_beta_n_, (size_t)nq * sizeof(float)))) return rc;
-  if ((rc = scratch(&s_fail_, &s_fail_n_, ((size_t)nq + 1) * sizeof(uint32_t)))) return rc;
+  // fail counts (one per range) then each range's fail list at nr + soff
+  if ((rc = scratch(&s_fail_, &s_fail_n_, ((size_t)nq + nr) * sizeof(uint32_t)))) return rc;
   if ((rc = scratch(&s_err_, &s_err_n_, (size_t)nq * sizeof(double)))) return rc;
   if (!h_fail_) HIPCHK(hipHostMalloc((void**)&h_fail_, sizeof(uint32_t)), "hipHostMalloc");
+  const size_t io_b = round_up(nr * sizeof(uint32_t), 64) + (size_t)nq * sizeof(double);
+  if (h_io_n_ < io_b) {
+    if (h_io_) (void)hipHostFree(h_io_);
+    h_io_ = nullptr;
+    h_io_n_ = 0;
+    HIPCHK(hipHostMalloc(&h_io_, round_up(io_b, 4096)), "hipHostMalloc");
+    h_io_n_ = round_up(io_b, 4096);
+  }
+  uint32_t* h_counts = static_cast<uint32_t*>(h_io_);
+  double* h_err = reinterpret_cast<double*>(static_cast<char*>(h_io_) + round_up(nr * sizeof(uint32_t), 64));
   if ((rc = sync_alive())) return rc;
 
   HIPCHK(hipMemsetAsync(s_err_, 0, (size_t)nq * sizeof(double), stream_), "memset");
@@ -790,13 +851,10 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     return fp8 ? launch_scan_fp8(x, m, nsl, minonly, stream_)
                : launch_scan_bf16(x, m, nsl, minonly, stream_);
   };
-  const int smetric = multi_metric ? kMixed : R[0].metric;
-  LowpScanArgs sa{fp8 ? (const uint16_t*)sh8_.X8 : sh_.Xb, dp,
-                  multi_metric ? sh_.nx : w0_of(R[0].metric),
-                  multi_metric ? sh_.pen : w1_of(R[0].metric),
-                  multi_metric ? sh_.xn : nullptr, multi_metric ? sh_.inv : nullptr,
-                  (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, rps, rps, nrows_pad, NL,
-                  (float*)s_ck_, (uint32_t*)s_cs_, nullptr, 0};
+  // the fields every launch shares; each sets its own rows, queries and lists
+  LowpScanArgs sa{fp8 ? (const uint16_t*)sh8_.X8 : sh_.Xb, dp, nullptr, nullptr, nullptr,
+                  nullptr, (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, 0, 0,
+                  nrows_pad, 0, (float*)s_ck_, (uint32_t*)s_cs_, nullptr, 0};
   if (pre) {
     prof_begin(SVDB_K_PREPASS);
     for (int i = 0; i < nr; i++) {
@@ -822,15 +880,33 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     sa.tau0_stride = 1;
   }
   prof_begin(SVDB_K_SCAN_LOWP);
-  HIPCHK(scan(sa, smetric, ns, false), "first-pass scan");
+  for (const ScanGroup& g : sgs) {
+    if (g.n == 0) continue;
+    const LowpRange& r = R[g.r0];
+    const bool mk = g.metric == kMixed;
+    LowpScanArgs ma = sa;
+    ma.w0 = mk ? sh_.nx : w0_of(g.metric);
+    ma.w1 = mk ? sh_.pen : w1_of(g.metric);
+    ma.w2 = mk ? sh_.xn : nullptr;
+    ma.w3 = mk ? sh_.inv : nullptr;
+    ma.Qb = (const uint16_t*)((const unsigned char*)s_qb_ + (size_t)g.soff * dp * qbytes);
+    ma.qp = (const QParams*)s_qp_ + g.soff;
+    ma.nq = g.n;
+    ma.rows_per_slice = ma.slice_stride = r.rps;
+    ma.NL = r.NL;
+    ma.cand_key = (float*)s_ck_ + r.coff;
+    ma.cand_slot = (uint32_t*)s_cs_ + r.coff;
+    if (ma.tau0) ma.tau0 += g.soff;
+    HIPCHK(scan(ma, g.metric, r.ns, false), "first-pass scan");
+  }
   prof_end(SVDB_K_SCAN_LOWP);
   // range i's shortlist: width Li = width(k_i), at slot offset soff (row
   // stride Li inside the range's block of soff*L .. )
   prof_begin(SVDB_K_SHORTLIST);
   for (int i = 0; i < nr; i++) {
     const uint32_t Li = width(R[i].k);
-    ShortlistArgs sl{(const float*)s_ck_ + (size_t)R[i].soff * NL * kLowpLP,
-                     (const uint32_t*)s_cs_ + (size_t)R[i].soff * NL * kLowpLP, R[i].n, NL, Li,
+    ShortlistArgs sl{(const float*)s_ck_ + R[i].coff, (const uint32_t*)s_cs_ + R[i].coff,
+                     R[i].n, R[i].NL, Li,
                      (uint32_t*)s_ss_ + (size_t)R[i].soff * L, (float*)s_sk_ + (size_t)R[i].soff * L,
                      (float*)s_beta_ + R[i].soff};
     HIPCHK(launch_shortlist(sl, stream_), "k_shortlist");
@@ -838,34 +914,55 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   prof_end(SVDB_K_SHORTLIST);
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
   const uint32_t Lmax = L;  // shortlist block stride per slot
+  // FP64 rerank of every range's shortlist; fail counts and the first-pass
+  // error report come back together after one wait
+  HIPCHK(hipMemsetAsync(s_fail_, 0, nr * sizeof(uint32_t), stream_), "memset");
+  auto rerank_args = [&](int ri) {
+    const LowpRange& rg = R[ri];
+    const uint32_t* fl = (uint32_t*)s_fail_ + nr + rg.soff;
+    return RerankArgs{sv, dQ + (size_t)rg.qoff * ldq, ldq, (uint32_t)d_,
+                      (const QParams*)s_qp_ + rg.soff,
+                      (const uint32_t*)s_ss_ + (size_t)rg.soff * Lmax,
+                      (const float*)s_sk_ + (size_t)rg.soff * Lmax,
+                      (const float*)s_beta_ + rg.soff, degen, degen_count, w0_of(rg.metric),
+                      w1_of(rg.metric), rg.n, width(rg.k), (uint32_t)rg.k, ostride,
+                      dids + (size_t)rg.qoff * ostride, dscores + (size_t)rg.qoff * ostride,
+                      (uint32_t*)s_fail_ + ri, const_cast<uint32_t*>(fl),
+                      (double*)s_err_ + rg.soff};
+  };
+  prof_begin(SVDB_K_RERANK);
+  for (int ri = 0; ri < nr; ri++) {
+    if (R[ri].n == 0) continue;
+    RerankArgs ra = rerank_args(ri);
+    HIPCHK(launch_rerank(ra, R[ri].metric, stream_), "k_rerank");
+  }
+  prof_end(SVDB_K_RERANK);
+  HIPCHK(hipMemcpyAsync(h_counts, s_fail_, nr * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+         "copy");
+  HIPCHK(hipMemcpyAsync(h_err, s_err_, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost, stream_),
+         "copy");
+  HIPCHK(hipStreamSynchronize(stream_), "lowp");
   double mx = 0.0;
+  for (uint32_t i = 0; i < nq; i++) mx = std::max(mx, h_err[i]);
+  std::vector<uint32_t> nfail(h_counts, h_counts + nr);
   for (int ri = 0; ri < nr; ri++) {
     const LowpRange& rg = R[ri];
     const int metric = rg.metric;
     const uint32_t nqr = rg.n;
     const int k = rg.k;                // this range's k (shadows the call's)
     const uint32_t L = width(k);       // and its shortlist width
-    if (nqr == 0) continue;
+    const uint32_t NL = rg.NL;         // and its candidate lists per query
+    const uint32_t nf1 = nfail[ri];
+    st_.certified += nqr - nf1;
+    if (nf1 == 0) continue;
     // this range's views: slot-space arrays at soff, query-space at qoff
     const double* rQ = dQ + (size_t)rg.qoff * ldq;
     uint64_t* rids = dids + (size_t)rg.qoff * ostride;
     double* rsc = dscores + (size_t)rg.qoff * ostride;
-    const float* rck = (const float*)s_ck_ + (size_t)rg.soff * NL * kLowpLP;
-    const uint32_t* rcs = (const uint32_t*)s_cs_ + (size_t)rg.soff * NL * kLowpLP;
-    uint32_t* fail_count = (uint32_t*)s_fail_;
-    uint32_t* fail_list = fail_count + 1;  // may be redirected to a widening retry's list
-    HIPCHK(hipMemsetAsync(fail_count, 0, sizeof(uint32_t), stream_), "memset");
-    RerankArgs ra{sv, rQ, ldq, (uint32_t)d_, (const QParams*)s_qp_ + rg.soff,
-                  (const uint32_t*)s_ss_ + (size_t)rg.soff * Lmax,
-                  (const float*)s_sk_ + (size_t)rg.soff * Lmax, (const float*)s_beta_ + rg.soff,
-                  degen, degen_count, w0_of(metric), w1_of(metric), nqr, L, (uint32_t)k,
-                  ostride, rids, rsc, fail_count, fail_list, (double*)s_err_ + rg.soff};
-    prof_begin(SVDB_K_RERANK);
-    HIPCHK(launch_rerank(ra, metric, stream_), "k_rerank");
-    prof_end(SVDB_K_RERANK);
-    HIPCHK(hipMemcpyAsync(h_fail_, fail_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "copy");
-    HIPCHK(hipStreamSynchronize(stream_), "lowp");
-    const uint32_t nf1 = *h_fail_;
+    const float* rck = (const float*)s_ck_ + rg.coff;
+    const uint32_t* rcs = (const uint32_t*)s_cs_ + rg.coff;
+    const RerankArgs ra = rerank_args(ri);
+    uint32_t* fail_list = ra.fail_list;  // may be redirected to a widening retry's list
     uint32_t nf = nf1;
     // Widening retries: queries not certified at L are shortlisted again at
     // 1024, then 4096, from the same candidate lists (when they hold at
@@ -905,12 +1002,16 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
       nf = *h_fail_;
       fail_list = out_list;
     }
-    // first-pass error report over the queries certified so far
-    std::vector<double> err(nqr);
-    HIPCHK(hipMemcpy(err.data(), (double*)s_err_ + rg.soff, nqr * sizeof(double),
-                     hipMemcpyDeviceToHost), "copy");
-    for (double e : err) mx = std::max(mx, e);
-    st_.certified += nqr - nf;
+    // the error report again, now with the widened queries certified
+    if (nf != nf1) {
+      double* he = reinterpret_cast<double*>(static_cast<char*>(h_io_) +
+                                             round_up(nr * sizeof(uint32_t), 64));
+      HIPCHK(hipMemcpyAsync(he, (double*)s_err_ + rg.soff, nqr * sizeof(double),
+                            hipMemcpyDeviceToHost, stream_), "copy");
+      HIPCHK(hipStreamSynchronize(stream_), "copy");
+      for (uint32_t i = 0; i < nqr; i++) mx = std::max(mx, he[i]);
+    }
+    st_.certified += nf1 - nf;
     st_.widened += nf1 - nf;
     if (nf == 0) continue;
     // Second chance for FP8: queries its stages leave uncertified go through
@@ -1106,13 +1207,14 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
       psc = hsc.data();
     }
   };
-  // Several metrics in one BF16 batch (rows <= 256 wide): one mixed-metric
-  // first pass for all of them, queries grouped by metric.
+  // Several metrics in one BF16 batch (rows <= 256 wide): one first pass
+  // for all of them, queries grouped by metric.
   if (which == 1 && sh_.dp <= 256 && nq <= step) {
     {
-      // one range per metric, all at the batch's largest k (splitting by k
-      // as well costs more in latency-bound per-range shortlists than the
-      // narrower lists save)
+      // one range per metric, all at the batch's largest k: a wavefront of
+      // k > 32 queries in a block sets that block's pace anyway, and extra
+      // ranges cost more in latency-bound shortlist and rerank launches than
+      // their narrower lists save
       LowpRange R[3];
       int nr = 0;
       sel.clear();
@@ -1122,7 +1224,7 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
           if (metric[i] == m) sel.push_back(i);
         if (sel.size() > first)
           R[nr++] = LowpRange{m, (uint32_t)first, (uint32_t)(sel.size() - first),
-                              (int)kmax_all, 0};
+                              (int)kmax_all, 0, 0, 0, 0, 0};
       }
       if (nr < 2) goto per_metric;  // one metric: the plain path below
       const size_t n = sel.size();

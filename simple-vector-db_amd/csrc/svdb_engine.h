@@ -117,6 +117,10 @@ class Engine {
     uint32_t qoff, n;  // query rows qoff .. qoff+n-1
     int k;             // neighbours wanted (0: the call's k)
     uint32_t soff;     // first slot (set by run_lowp_ranges)
+    // first-pass row slicing of this range's scan and where its candidate
+    // lists start (set by run_lowp_ranges)
+    uint32_t ns, rps, NL;
+    size_t coff;
   };
   int run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k, uint64_t* dids,
                double* dscores, uint32_t ostride, bool fp8);
@@ -196,6 +200,8 @@ class Engine {
   void* s_cid_ = nullptr;    size_t s_cid_n_ = 0;   //   their ids
   void* s_csc_ = nullptr;    size_t s_csc_n_ = 0;   //   and scores
   uint32_t* h_fail_ = nullptr;  // pinned: fail count
+  void* h_io_ = nullptr;        // pinned: per-range fail counts + first-pass errors
+  size_t h_io_n_ = 0;
   void* h_stage_ = nullptr;     // pinned: row upload and search staging
   size_t h_stage_n_ = 0;
 

@@ -492,13 +492,20 @@ hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s) {
   if (a.nq == 0) return hipSuccess;
   if (a.L == 0 || a.L > (uint32_t)kMaxShort) return hipErrorInvalidValue;
   const dim3 g((a.nq + 3) / 4), b(256);
+  // Up to 64 wide: one wavefront per query keeps a sorted list (few
+  // insertions past the first 64).  Wider: the block-per-query select,
+  // whose cost does not grow with L (a 256-wide sorted list in one
+  // wavefront takes hundreds of serial insertions per query); the sorted
+  // 256-wide form remains for candidate sets too large for its registers.
+  const bool sel_fits = (size_t)a.NL * kLowpLP <= 256u * kSelPT;
   if (a.L <= 64) {
     hipLaunchKernelGGL(k_shortlist<1>, g, b, 0, s, a);
+  } else if (sel_fits) {
+    hipLaunchKernelGGL(k_shortlist_sel, dim3(a.nq), b, 0, s, a);
   } else if (a.L <= 256) {
     hipLaunchKernelGGL(k_shortlist<4>, g, b, 0, s, a);
   } else {
-    if ((size_t)a.NL * kLowpLP > 256u * kSelPT) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_shortlist_sel, dim3(a.nq), b, 0, s, a);
+    return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
@@ -669,6 +676,30 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
     }
     return;
   }
+  // First-pass keys of this lane's results, for the error report: the
+  // shortlist passes through the wavefront's LDS area in chunks of kOC
+  // (slot, key) pairs and every lane matches its two result slots there.
+  constexpr uint32_t kOC = 1024;  // 8 KB of the 64 * (kRC + 1) doubles
+  static_assert(kOC * 8 <= 64 * (kRC + 1) * sizeof(double), "chunk exceeds the stage");
+  float O2[2] = {INFINITY, INFINITY};
+  {
+    uint32_t* oslot = reinterpret_cast<uint32_t*>(stg);
+    float* okey = reinterpret_cast<float*>(oslot + kOC);
+    for (uint32_t t0 = 0; t0 < a.L; t0 += kOC) {
+      const uint32_t nt = min(kOC, a.L - t0);
+      for (uint32_t t = lane; t < nt; t += 64) {
+        oslot[t] = a.short_slot[(size_t)i * a.L + t0 + t];
+        okey[t] = a.short_key[(size_t)i * a.L + t0 + t];
+      }
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t t = 0; t < nt; t++) {
+        const uint32_t sl = oslot[t];
+        if (sl == L.s[0]) O2[0] = okey[t];
+        if (sl == L.s[1]) O2[1] = okey[t];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
   double err = 0.0;
 #pragma unroll
   for (int p = 0; p < 2; p++) {
@@ -685,9 +716,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
                        (uint64_t)__popcll(a.st.alive[slot >> 6] & ((1ull << (slot & 63)) - 1ull))
                  : slot;
         // first-pass estimate of this score, for the error report
-        float O = INFINITY;
-        for (uint32_t t = 0; t < a.L; t++)
-          if (a.short_slot[(size_t)i * a.L + t] == slot) O = a.short_key[(size_t)i * a.L + t];
+        const float O = O2[p];
         if (O != INFINITY && sc == sc && sc != 0.0) {
           const double xn = a.st.xn[slot];
           double est;

@@ -127,6 +127,19 @@ def test_mixed_unbalanced_ranges(mix):
     _check_mixed(X, Qm, ks, ms, ids, sc)
 
 
+@pytest.mark.parametrize("metric", ["l2", "dot"])
+def test_single_metric_k_classes(mix, metric):
+    """One metric, k on both sides of 32 and the largest k allowed in one
+    call (one range at the largest k, each query's own k returned)."""
+    X, Q, db = mix
+    rng = np.random.default_rng(8)
+    ks = rng.choice([1, 5, 33, 50], size=600)
+    ks[17] = 128
+    say(f"{metric}: k classes <=32 and >32 in one call")
+    ids, sc = db.search(Q[:600], ks, metric, "auto")
+    _check_mixed(X, Q[:600], ks, [metric] * 600, ids, sc)
+
+
 def test_bf16_after_mutations_and_degenerate_rows():
     rng = np.random.default_rng(12)
     d = 96  # padded to 128 in the shadow
