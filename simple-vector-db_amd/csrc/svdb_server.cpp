@@ -8,7 +8,8 @@
 // engine (one first-pass scan per metric present).  Requests that arrive
 // while a tick is on the GPU wait for the next tick; the queries of the
 // next tick are staged (copied and grouped on the host) by a second thread
-// while the current tick runs.  A mutation is a barrier: searches queued
+// while the current tick runs, and searches keep joining it until the
+// device thread takes it.  A mutation is a barrier: searches queued
 // before it see the store without it, searches after it see it.  The host
 // mirror (db->vectors) follows the device store on a third thread, in the
 // same order; a mutation completes once both halves are applied.
@@ -53,7 +54,7 @@ struct Tick {
   std::vector<double> q;
   std::vector<size_t> k;
   std::vector<int> metric;
-  size_t kmax = 0, nq = 0;
+  size_t kmax = 0, nq = 0, bytes = 0;
   // A tick is either a run of consecutive mutations (applied in order) or
   // a batch of searches.
   std::vector<std::shared_ptr<Request>> mutations;
@@ -144,63 +145,70 @@ class Server {
   size_t dim() const { return eng_->dim(); }
 
  private:
-  // Stager: forms the next tick while the executor runs the current one.
+  // Stager: forms the next tick while the executor runs the current one,
+  // and keeps adding the searches that arrive meanwhile to it (while they
+  // fit) until the executor takes it; a search then waits at most for the
+  // tick that is running when it arrives.
   void stage_loop() {
+    std::unique_lock<std::mutex> g(mu_);
     for (;;) {
-      std::unique_ptr<Tick> t(new Tick);
-      {
-        std::unique_lock<std::mutex> g(mu_);
-        cv_q_.wait(g, [&] { return stop_ || (!queue_.empty() && !staged_); });
-        if (stop_) return;
-        if (queue_.front()->kind != 0) {
-          while (!queue_.empty() && queue_.front()->kind != 0 &&
-                 t->mutations.size() < kMaxMutationsPerTick) {
-            t->mutations.push_back(queue_.front());
-            queue_.pop_front();
-          }
-        } else {
-          size_t bytes = 0;
-          while (!queue_.empty() && queue_.front()->kind == 0) {
-            auto r = queue_.front();
-            if (r->dim != eng_->dim()) {  // store dimension changed under it
-              queue_.pop_front();
-              r->rc = SVDB_EINVAL;
-              r->done = true;
-              cv_done_.notify_all();
-              continue;
-            }
-            if (!t->reqs.empty() &&
-                (t->nq + r->nq > cfg_.max_batch || bytes + r->est_bytes > cfg_.budget_bytes))
-              break;
-            bytes += r->est_bytes;
-            t->nq += r->nq;
-            t->reqs.push_back(r);
-            queue_.pop_front();
-          }
+      cv_q_.wait(g, [&] { return stop_ || can_stage(); });
+      if (stop_) return;
+      if (!staged_ && queue_.front()->kind != 0) {
+        std::unique_ptr<Tick> t(new Tick);
+        while (!queue_.empty() && queue_.front()->kind != 0 &&
+               t->mutations.size() < kMaxMutationsPerTick) {
+          t->mutations.push_back(queue_.front());
+          queue_.pop_front();
         }
-      }
-      if (t->mutations.empty()) {
-        const size_t d = eng_->dim();
-        t->q.resize(t->nq * d);
-        t->k.resize(t->nq);
-        t->metric.resize(t->nq);
-        size_t o = 0;
-        for (auto& r : t->reqs) {
-          std::memcpy(&t->q[o * d], r->data.data(), r->nq * d * sizeof(double));
-          for (size_t i = 0; i < r->nq; i++) {
-            t->k[o + i] = r->k;
-            t->metric[o + i] = r->metric;
-          }
-          t->kmax = std::max(t->kmax, r->k);
-          o += r->nq;
-        }
-      }
-      {
-        std::lock_guard<std::mutex> g(mu_);
         staged_ = std::move(t);
+      } else {
+        std::unique_ptr<Tick> fresh;
+        Tick* t = staged_.get();
+        if (!t) {
+          fresh.reset(new Tick);
+          t = fresh.get();
+        }
+        const size_t d = eng_->dim();
+        while (!queue_.empty() && queue_.front()->kind == 0) {
+          auto r = queue_.front();
+          if (r->dim != d) {  // store dimension changed under it
+            queue_.pop_front();
+            r->rc = SVDB_EINVAL;
+            r->done = true;
+            cv_done_.notify_all();
+            continue;
+          }
+          if (!t->reqs.empty() &&
+              (t->nq + r->nq > cfg_.max_batch || t->bytes + r->est_bytes > cfg_.budget_bytes))
+            break;
+          queue_.pop_front();
+          const size_t o = t->nq;
+          t->q.resize((o + r->nq) * d);
+          std::memcpy(&t->q[o * d], r->data.data(), r->nq * d * sizeof(double));
+          t->k.resize(o + r->nq, r->k);
+          t->metric.resize(o + r->nq, r->metric);
+          t->kmax = std::max(t->kmax, r->k);
+          t->nq += r->nq;
+          t->bytes += r->est_bytes;
+          t->reqs.push_back(r);
+        }
+        if (fresh && !fresh->reqs.empty()) staged_ = std::move(fresh);
       }
       cv_tick_.notify_all();
     }
+  }
+
+  // (under mu_) whether the stager has work: a queue head that starts a new
+  // tick, or a search that joins the staged search tick
+  bool can_stage() const {
+    if (queue_.empty()) return false;
+    if (!staged_) return true;
+    const Tick& t = *staged_;
+    const Request& r = *queue_.front();
+    if (!t.mutations.empty() || r.kind != 0) return false;
+    if (r.dim != eng_->dim()) return true;
+    return t.nq + r.nq <= cfg_.max_batch && t.bytes + r.est_bytes <= cfg_.budget_bytes;
   }
 
   void exec_loop() {
