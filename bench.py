@@ -241,10 +241,10 @@ def serve(srv, plan, pool, rate=None):
             "mutation_ms": round(float(tick_us[~srch].mean()) * 1e-3, 3) if (~srch).any() else None}
 
 
-def serving_sweep(db, pool, store_rows, seconds, loads=(0.3, 0.5, 0.7, 0.9)):
+def serving_sweep(db, pool, store_rows, seconds, loads=(0.3, 0.5, 0.7, 0.9), max_batch=8192):
     import svdb
     rng = np.random.default_rng(77)
-    srv = svdb.Server(db, max_batch=8192, budget_bytes=64 << 30, precision="auto")
+    srv = svdb.Server(db, max_batch=max_batch, budget_bytes=64 << 30, precision="auto")
     serve(srv, serving_plan(rng, 200, pool.shape[0], store_rows), pool)  # warm
     db.set_profiling(True)
     s0 = db.stats()
@@ -261,7 +261,7 @@ def serving_sweep(db, pool, store_rows, seconds, loads=(0.3, 0.5, 0.7, 0.9)):
     sat["kernel_launches"] = {name: int(kt[i][0]) for i, name in
                               ((svdb.K_SCAN_LOWP, "scan_bf16"), (svdb.K_SCAN_FP64, "scan_fp64"))}
     db.set_profiling(False)
-    log(f"serving saturation: {sat}")
+    log(f"serving saturation (max_batch {max_batch}): {sat}")
     levels = []
     for f in loads:
         rate = f * sat["req_s"]
@@ -269,7 +269,7 @@ def serving_sweep(db, pool, store_rows, seconds, loads=(0.3, 0.5, 0.7, 0.9)):
         r = serve(srv, plan, pool, rate)
         r["load"] = f
         r["offered_req_s"] = round(rate, 1)
-        log(f"serving at {f:.0%}: {r}")
+        log(f"serving at {f:.0%} (max_batch {max_batch}): {r}")
         levels.append(r)
     srv.stop()
     return sat, levels
@@ -554,6 +554,11 @@ def main():
     if not args.no_serving:
         sat, levels = serving_sweep(db, Q, args.rows, args.serving_seconds)
         at70 = next((lv for lv in levels if abs(lv["load"] - 0.7) < 1e-9), None)
+        # the same workload with ticks capped at 1024 queries: lower
+        # saturated throughput, shorter ticks, lower tail latency
+        sat_l, levels_l = serving_sweep(db, Q, args.rows, args.serving_seconds,
+                                        loads=(0.5, 0.7), max_batch=1024)
+        at70_l = next((lv for lv in levels_l if abs(lv["load"] - 0.7) < 1e-9), None)
         serving = {
             "workload": "open-loop Poisson arrivals; batch log-uniform 1..512 queries, "
                         "k in {1,10,100}, metric uniform, 5% mutations; max_batch 8192, "
@@ -561,6 +566,11 @@ def main():
             "saturation": sat, "levels": levels,
             "p50_ms_at_70": at70["p50_ms"] if at70 else None,
             "p99_ms_at_70": at70["p99_ms"] if at70 else None,
+            "max_batch_1024": {
+                "saturation_queries_s": sat_l["queries_s"], "levels": levels_l,
+                "p50_ms_at_70": at70_l["p50_ms"] if at70_l else None,
+                "p99_ms_at_70": at70_l["p99_ms"] if at70_l else None,
+            },
         }
 
     lat_sorted = np.sort(np.array(lat))

@@ -318,20 +318,26 @@ __global__ __launch_bounds__(256) void k_scan_f64(ScanF64Args a) {
         s1 = -(s1 / (qn[j] * xn1));
       }
       const uint64_t k0 = rank_bits(s0), k1 = rank_bits(s1);
-      uint64_t m0 = __ballot(v0 && lex_lt(k0, r0, tk[j], ts[j]));
-      uint64_t m1 = __ballot(v1 && lex_lt(k1, r1, tk[j], ts[j]));
-      if (m0 | m1) {
-        while (m0) {
-          int src = __ffsll((long long)m0) - 1;
-          m0 &= m0 - 1;
-          L[j].insert(uniform64(__shfl(k0, src)), base + src, kq[j], lane);
-        }
-        while (m1) {
-          int src = __ffsll((long long)m1) - 1;
-          m1 &= m1 - 1;
-          L[j].insert(uniform64(__shfl(k1, src)), base + 64 + src, kq[j], lane);
-        }
+      // slots past nq (a partial group) keep no list
+      const bool jv = q0 + j < a.nq;
+      uint64_t m0 = __ballot(jv && v0 && lex_lt(k0, r0, tk[j], ts[j]));
+      uint64_t m1 = __ballot(jv && v1 && lex_lt(k1, r1, tk[j], ts[j]));
+      // After each insertion the k-th entry may have moved: lanes that no
+      // longer beat it drop out (their insertion would change nothing).
+      while (m0) {
+        int src = __ffsll((long long)m0) - 1;
+        m0 &= m0 - 1;
+        L[j].insert(uniform64(__shfl(k0, src)), base + src, kq[j], lane);
         L[j].kth(kq[j], tk[j], ts[j]);
+        m0 &= __ballot(lex_lt(k0, r0, tk[j], ts[j]));
+      }
+      m1 &= __ballot(lex_lt(k1, r1, tk[j], ts[j]));
+      while (m1) {
+        int src = __ffsll((long long)m1) - 1;
+        m1 &= m1 - 1;
+        L[j].insert(uniform64(__shfl(k1, src)), base + 64 + src, kq[j], lane);
+        L[j].kth(kq[j], tk[j], ts[j]);
+        m1 &= __ballot(lex_lt(k1, r1, tk[j], ts[j]));
       }
     }
   }
@@ -406,6 +412,8 @@ __global__ __launch_bounds__(256) void k_merge(MergeArgs a) {
         int src = __ffsll((long long)m) - 1;
         m &= m - 1;
         L.insert(uniform64(__shfl(ck, src)), uniform32(__shfl(cs, src)), kk, lane);
+        L.kth(kk, tk, ts);
+        m &= __ballot(lex_lt(ck, cs, tk, ts));
       }
     }
     L.kth(kk, tk, ts);

@@ -335,6 +335,9 @@ __global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
         int src = __ffsll((long long)m) - 1;
         m &= m - 1;
         L.insert(uni64(__shfl(K, src)), uni32(__shfl(slot[u], src)), kk, lane);
+        // lanes that no longer beat the (moved) kk-th entry drop out
+        L.at(kk - 1, tk, ts);
+        m &= __ballot(lt2(K, slot[u], tk, ts));
       }
       L.at(kk - 1, tk, ts);
     }
@@ -648,6 +651,9 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
       int src = __ffsll((long long)m) - 1;
       m &= m - 1;
       L.insert(uni64(__shfl(K, src)), uni32(__shfl(slot, src)), kk, lane);
+      // lanes that no longer beat the (moved) k-th entry drop out
+      L.at(kk - 1, tk, ts);
+      m &= __ballot(lt2(K, slot, tk, ts));
     }
     L.at(kk - 1, tk, ts);
   }
