@@ -133,7 +133,9 @@ int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_C
  * within budget_bytes) into one batch; mutations are applied between ticks
  * in order (a search sees exactly the mutations submitted before it).
  * Requests arriving during a tick join the next one, which is staged on
- * the host while the current tick runs.  Results equal those of svdb_search
+ * the host while the current tick runs and takes searches until the device
+ * thread starts it.  A smaller max_batch gives shorter ticks (lower tail
+ * latency) at some cost in saturated throughput.  Results equal those of svdb_search
  * on the same store state.  Stop the server only after every submitted
  * request has been waited for. */
 typedef struct svdb_server svdb_server;
