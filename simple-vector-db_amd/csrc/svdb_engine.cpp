@@ -902,15 +902,15 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   prof_end(SVDB_K_SCAN_LOWP);
   // range i's shortlist: width Li = width(k_i), at slot offset soff (row
   // stride Li inside the range's block of soff*L .. )
+  // every range's shortlist, in as few launches as the kernels allow
+  std::vector<ShortlistArgs> sls(nr);
+  for (int i = 0; i < nr; i++)
+    sls[i] = ShortlistArgs{(const float*)s_ck_ + R[i].coff, (const uint32_t*)s_cs_ + R[i].coff,
+                           R[i].n, R[i].NL, width(R[i].k),
+                           (uint32_t*)s_ss_ + (size_t)R[i].soff * L,
+                           (float*)s_sk_ + (size_t)R[i].soff * L, (float*)s_beta_ + R[i].soff};
   prof_begin(SVDB_K_SHORTLIST);
-  for (int i = 0; i < nr; i++) {
-    const uint32_t Li = width(R[i].k);
-    ShortlistArgs sl{(const float*)s_ck_ + R[i].coff, (const uint32_t*)s_cs_ + R[i].coff,
-                     R[i].n, R[i].NL, Li,
-                     (uint32_t*)s_ss_ + (size_t)R[i].soff * L, (float*)s_sk_ + (size_t)R[i].soff * L,
-                     (float*)s_beta_ + R[i].soff};
-    HIPCHK(launch_shortlist(sl, stream_), "k_shortlist");
-  }
+  HIPCHK(launch_shortlist_ranges(sls.data(), nr, stream_), "k_shortlist");
   prof_end(SVDB_K_SHORTLIST);
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
   const uint32_t Lmax = L;  // shortlist block stride per slot
@@ -930,12 +930,14 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
                       (uint32_t*)s_fail_ + ri, const_cast<uint32_t*>(fl),
                       (double*)s_err_ + rg.soff};
   };
-  prof_begin(SVDB_K_RERANK);
+  std::vector<RerankArgs> ras(nr);
+  std::vector<int> rms(nr);
   for (int ri = 0; ri < nr; ri++) {
-    if (R[ri].n == 0) continue;
-    RerankArgs ra = rerank_args(ri);
-    HIPCHK(launch_rerank(ra, R[ri].metric, stream_), "k_rerank");
+    ras[ri] = rerank_args(ri);
+    rms[ri] = R[ri].metric;
   }
+  prof_begin(SVDB_K_RERANK);
+  HIPCHK(launch_rerank_ranges(ras.data(), rms.data(), nr, stream_), "k_rerank");
   prof_end(SVDB_K_RERANK);
   HIPCHK(hipMemcpyAsync(h_counts, s_fail_, nr * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
          "copy");

@@ -185,6 +185,11 @@ hipError_t launch_scan_fp8(const LowpScanArgs& a, int metric, uint32_t nslices, 
 
 hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s);
 hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s);
+// n ranges in as few launches as possible (kMaxLaunchRanges per launch; for
+// the shortlist, consecutive ranges that take the same kernel)
+constexpr int kMaxLaunchRanges = 4;
+hipError_t launch_shortlist_ranges(const ShortlistArgs* a, int n, hipStream_t s);
+hipError_t launch_rerank_ranges(const RerankArgs* a, const int* metric, int n, hipStream_t s);
 hipError_t launch_gather_rows(const double* src, size_t lds, const uint32_t* idx, uint32_t n,
                               uint32_t d, double* dst, size_t ldd, hipStream_t s);
 hipError_t launch_scatter_results(const uint64_t* ids, const double* sc, uint32_t k,

@@ -295,9 +295,9 @@ hipError_t launch_query_prep(const double* Q, size_t ldq, uint32_t nq, uint32_t 
 // shortlist has a smaller ordering key.
 
 template <int KPL>
-__global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
+__device__ __forceinline__ void shortlist_body(const ShortlistArgs& a, uint32_t blk) {
   const int lane = threadIdx.x & 63;
-  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t i = blk * 4 + (threadIdx.x >> 6);
   if (i >= a.nq) return;
   const uint32_t q = a.qidx ? a.qidx[i] : i;  // candidate lists of query q -> output row i
   const uint32_t n = a.NL * kLowpLP;
@@ -371,12 +371,20 @@ __global__ __launch_bounds__(256) void k_shortlist(ShortlistArgs a) {
 // beta = min(8th entry of every list, K*) as in the sorted form.
 constexpr int kSelPT = 64;
 
-__global__ __launch_bounds__(256) void k_shortlist_sel(ShortlistArgs a) {
-  __shared__ uint32_t hist[256];
-  __shared__ uint32_t s_prefix, s_mask, s_rank, s_nvalid, s_lt, s_eq;
-  __shared__ float s_min[4];
+struct SelShared {
+  uint32_t hist[256];
+  uint32_t prefix, mask, rank, nvalid, lt, eq;
+  float min[4];
+};
+
+__device__ __forceinline__ void shortlist_sel_body(const ShortlistArgs& a, uint32_t blk,
+                                                   SelShared& sm) {
+  uint32_t* hist = sm.hist;
+  uint32_t &s_prefix = sm.prefix, &s_mask = sm.mask, &s_rank = sm.rank, &s_nvalid = sm.nvalid,
+           &s_lt = sm.lt, &s_eq = sm.eq;
+  float* s_min = sm.min;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t i = blockIdx.x;
+  const uint32_t i = blk;
   const uint32_t q = a.qidx ? a.qidx[i] : i;
   const uint32_t n = a.NL * kLowpLP;
   const size_t base = (size_t)q * n;
@@ -491,26 +499,79 @@ __global__ __launch_bounds__(256) void k_shortlist_sel(ShortlistArgs a) {
   }
 }
 
-hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s) {
-  if (a.nq == 0) return hipSuccess;
-  if (a.L == 0 || a.L > (uint32_t)kMaxShort) return hipErrorInvalidValue;
-  const dim3 g((a.nq + 3) / 4), b(256);
-  // Up to 64 wide: one wavefront per query keeps a sorted list (few
-  // insertions past the first 64).  Wider: the block-per-query select,
-  // whose cost does not grow with L (a 256-wide sorted list in one
-  // wavefront takes hundreds of serial insertions per query); the sorted
-  // 256-wide form remains for candidate sets too large for its registers.
+// Several ranges' shortlists (or reranks) in one launch: range r takes
+// blocks first[r] .. first[r+1]-1 (a launch per range costs its latency
+// once per range, and small ranges leave most of the GPU idle).
+template <typename A>
+struct RangeBatch {
+  A r[kMaxLaunchRanges];
+  int metric[kMaxLaunchRanges];
+  uint32_t first[kMaxLaunchRanges + 1];
+  int n;
+};
+
+template <typename A>
+__device__ __forceinline__ int range_of(const RangeBatch<A>& b, uint32_t blk) {
+  int i = 0;
+  while (i + 1 < b.n && blk >= b.first[i + 1]) i++;
+  return i;
+}
+
+template <int KPL>
+__global__ __launch_bounds__(256) void k_shortlist(RangeBatch<ShortlistArgs> b) {
+  const int i = range_of(b, blockIdx.x);
+  shortlist_body<KPL>(b.r[i], blockIdx.x - b.first[i]);
+}
+
+__global__ __launch_bounds__(256) void k_shortlist_sel(RangeBatch<ShortlistArgs> b) {
+  __shared__ SelShared sm;
+  const int i = range_of(b, blockIdx.x);
+  shortlist_sel_body(b.r[i], blockIdx.x - b.first[i], sm);
+}
+
+// Up to 64 wide: one wavefront per query keeps a sorted list (few
+// insertions past the first 64).  Wider: the block-per-query select, whose
+// cost does not grow with L (a 256-wide sorted list in one wavefront takes
+// hundreds of serial insertions per query); the sorted 256-wide form
+// remains for candidate sets too large for its registers.
+static int shortlist_variant(const ShortlistArgs& a) {
   const bool sel_fits = (size_t)a.NL * kLowpLP <= 256u * kSelPT;
-  if (a.L <= 64) {
-    hipLaunchKernelGGL(k_shortlist<1>, g, b, 0, s, a);
-  } else if (sel_fits) {
-    hipLaunchKernelGGL(k_shortlist_sel, dim3(a.nq), b, 0, s, a);
-  } else if (a.L <= 256) {
-    hipLaunchKernelGGL(k_shortlist<4>, g, b, 0, s, a);
-  } else {
-    return hipErrorInvalidValue;
+  if (a.L == 0 || a.L > (uint32_t)kMaxShort) return -1;
+  if (a.L <= 64) return 1;
+  if (sel_fits) return 0;
+  if (a.L <= 256) return 4;
+  return -1;
+}
+
+hipError_t launch_shortlist_ranges(const ShortlistArgs* a, int n, hipStream_t s) {
+  for (int r0 = 0; r0 < n;) {
+    // one launch per run of ranges that take the same kernel
+    RangeBatch<ShortlistArgs> b{};
+    const int v = shortlist_variant(a[r0]);
+    if (v < 0) return hipErrorInvalidValue;
+    uint32_t blocks = 0;
+    b.n = 0;
+    int r = r0;
+    for (; r < n && b.n < kMaxLaunchRanges && shortlist_variant(a[r]) == v; r++) {
+      b.r[b.n] = a[r];
+      b.first[b.n] = blocks;
+      blocks += v == 0 ? a[r].nq : (a[r].nq + 3) / 4;
+      b.n++;
+    }
+    b.first[b.n] = blocks;
+    r0 = r;
+    if (blocks == 0) continue;
+    if (v == 1) hipLaunchKernelGGL(k_shortlist<1>, dim3(blocks), dim3(256), 0, s, b);
+    else if (v == 4) hipLaunchKernelGGL(k_shortlist<4>, dim3(blocks), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(k_shortlist_sel, dim3(blocks), dim3(256), 0, s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
+}
+
+hipError_t launch_shortlist(const ShortlistArgs& a, hipStream_t s) {
+  return launch_shortlist_ranges(&a, 1, s);
 }
 
 // ------------------------------------------------ L-th smallest of n keys
@@ -575,12 +636,10 @@ hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L,
 constexpr int kRC = 16;
 
 template <int METRIC>
-__global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
-  __shared__ double stage[4][64 * (kRC + 1)];
+__device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, double* stg) {
   const int lane = threadIdx.x & 63;
-  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t i = blk * 4 + (threadIdx.x >> 6);
   if (i >= a.nq) return;
-  double* stg = stage[threadIdx.x >> 6];
   // shortlist row i belongs to query q (its parameters, FP64 row, outputs)
   const uint32_t q = a.qidx ? a.qidx[i] : i;
   const QParams qp = a.qp[q];
@@ -746,16 +805,43 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   if (lane == 0) a.max_rel_err[q] = err;
 }
 
-hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s) {
-  if (a.nq == 0) return hipSuccess;
-  dim3 grid((a.nq + 3) / 4), block(256);
-  switch (metric) {
-    case kL2: hipLaunchKernelGGL(k_rerank<kL2>, grid, block, 0, s, a); break;
-    case kCos: hipLaunchKernelGGL(k_rerank<kCos>, grid, block, 0, s, a); break;
-    case kDot: hipLaunchKernelGGL(k_rerank<kDot>, grid, block, 0, s, a); break;
-    default: return hipErrorInvalidValue;
+// one wavefront per shortlist row; each range its own metric
+__global__ __launch_bounds__(256) void k_rerank(RangeBatch<RerankArgs> b) {
+  __shared__ double stage[4][64 * (kRC + 1)];
+  const int i = range_of(b, blockIdx.x);
+  const uint32_t blk = blockIdx.x - b.first[i];
+  double* stg = stage[threadIdx.x >> 6];
+  switch (b.metric[i]) {
+    case kL2: rerank_body<kL2>(b.r[i], blk, stg); break;
+    case kCos: rerank_body<kCos>(b.r[i], blk, stg); break;
+    default: rerank_body<kDot>(b.r[i], blk, stg); break;
   }
-  return hipGetLastError();
+}
+
+hipError_t launch_rerank_ranges(const RerankArgs* a, const int* metric, int n, hipStream_t s) {
+  for (int r0 = 0; r0 < n; r0 += kMaxLaunchRanges) {
+    RangeBatch<RerankArgs> b{};
+    uint32_t blocks = 0;
+    b.n = 0;
+    for (int r = r0; r < n && b.n < kMaxLaunchRanges; r++) {
+      if (metric[r] != kL2 && metric[r] != kCos && metric[r] != kDot) return hipErrorInvalidValue;
+      b.r[b.n] = a[r];
+      b.metric[b.n] = metric[r];
+      b.first[b.n] = blocks;
+      blocks += (a[r].nq + 3) / 4;
+      b.n++;
+    }
+    b.first[b.n] = blocks;
+    if (blocks == 0) continue;
+    hipLaunchKernelGGL(k_rerank, dim3(blocks), dim3(256), 0, s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s) {
+  return launch_rerank_ranges(&a, &metric, 1, s);
 }
 
 // ------------------------------------------------------------- fallback plumbing
