@@ -238,9 +238,10 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
       // fill the tiles.  L2 and dot at widths <= 128: the A fragments of the next subtile are
       // read into the registers of this subtile's as soon as the MFMAs that
       // use them have issued, so their LDS latency runs under the epilogue
-      // test rather than before the next MFMAs (measured 10 % on Euclidean,
-      // 3 % on dot at 128 wide); cosine and the mixed scan have no
-      // registers for it and read each subtile's fragments at its start.
+      // test rather than before the next MFMAs (measured 1.7 % on dot at
+      // 128 wide, same box; Euclidean within its run-to-run spread);
+      // cosine and the mixed scan have no registers for it and read each
+      // subtile's fragments at its start.
       constexpr bool PFA = KS <= 4 && (METRIC == kL2 || METRIC == kDot);
       if constexpr (PFA) {
         bf16x8 A[KS];
