@@ -795,11 +795,15 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   // filling its lists (the insertion path costs several times the MFMAs).
   const uint32_t ns_pre = tiles >= 16 * kPrepassSlices ? kPrepassSlices : kPrepassSlices / 4;
   const bool pre = prepass_ && tiles >= 16 * ns_pre && ns_pre * kLowpLanesPerQuery >= L;
-  // Twice the sample for shortlists past 64 (their threshold sits further
-  // out, and a sparser sample leaves more rows below it to insert) and for
-  // batches of up to 4096 queries (the scan's saving outweighs the larger
-  // pre-pass there; at 16,384 queries and L = 64 they break even).
-  const uint32_t tpmax = (L > 64 || nq <= 4096) ? 2 * kPrepassMaxTiles : kPrepassMaxTiles;
+  // BF16: twice the sample for shortlists past 64 (their threshold sits
+  // further out, and a sparser sample leaves more rows below it to insert)
+  // and for batches of up to 4096 queries (the scan's saving outweighs the
+  // larger pre-pass there; at 16,384 queries and L = 64 they break even).
+  // Not FP8: a list the threshold leaves short bounds β by the threshold,
+  // and FP8's wider error band then certifies fewer queries (measured on
+  // 5M x 768 cosine: 861 instead of 4,293 of 12,288 before the BF16 pass).
+  const uint32_t tpmax =
+      (!fp8 && (L > 64 || nq <= 4096)) ? 2 * kPrepassMaxTiles : kPrepassMaxTiles;
   const uint32_t tp = pre ? std::min<uint32_t>(tpmax, tiles / (16 * ns_pre)) : 0;
   const uint32_t NL_pre = ns_pre * kLowpLanesPerQuery;
   const size_t ncand = std::max(cand_end, pre ? (size_t)nq * NL_pre : (size_t)0);

@@ -818,22 +818,37 @@ __global__ __launch_bounds__(256) void k_rerank(RangeBatch<RerankArgs> b) {
   }
 }
 
+// ranges of one metric: that metric's code only (fewer registers than the
+// kernel that carries all three)
+template <int METRIC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rerank1(
+    RangeBatch<RerankArgs> b) {
+  __shared__ double stage[4][64 * (kRC + 1)];
+  const int i = range_of(b, blockIdx.x);
+  rerank_body<METRIC>(b.r[i], blockIdx.x - b.first[i], stage[threadIdx.x >> 6]);
+}
+
 hipError_t launch_rerank_ranges(const RerankArgs* a, const int* metric, int n, hipStream_t s) {
   for (int r0 = 0; r0 < n; r0 += kMaxLaunchRanges) {
     RangeBatch<RerankArgs> b{};
     uint32_t blocks = 0;
     b.n = 0;
+    bool one = true;
     for (int r = r0; r < n && b.n < kMaxLaunchRanges; r++) {
       if (metric[r] != kL2 && metric[r] != kCos && metric[r] != kDot) return hipErrorInvalidValue;
       b.r[b.n] = a[r];
       b.metric[b.n] = metric[r];
       b.first[b.n] = blocks;
       blocks += (a[r].nq + 3) / 4;
+      one &= metric[r] == metric[r0];
       b.n++;
     }
     b.first[b.n] = blocks;
     if (blocks == 0) continue;
-    hipLaunchKernelGGL(k_rerank, dim3(blocks), dim3(256), 0, s, b);
+    if (!one) hipLaunchKernelGGL(k_rerank, dim3(blocks), dim3(256), 0, s, b);
+    else if (metric[r0] == kL2) hipLaunchKernelGGL(k_rerank1<kL2>, dim3(blocks), dim3(256), 0, s, b);
+    else if (metric[r0] == kCos) hipLaunchKernelGGL(k_rerank1<kCos>, dim3(blocks), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(k_rerank1<kDot>, dim3(blocks), dim3(256), 0, s, b);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
