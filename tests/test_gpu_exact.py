@@ -123,6 +123,16 @@ def test_kd_projection_columns(gauss):
     got = np.array([db7.kd_nearest(q) for q in Q[:40]], dtype=np.uint64)
     rid, _ = O.topk(X[:50_000], Q[:40], 1, "l2", D=7)
     assert np.array_equal(got, rid[:, 0])
+    # the same projection as one batched call: Euclidean over a column
+    # prefix (precision "auto" takes the FP64 scan for a prefix)
+    rid, rsc = O.topk(X[:50_000], Q[:40], 5, "l2", D=7)
+    for prec in ("auto", "fp64"):
+        ids, sc = db7.search(Q[:40, :7], 5, "l2", prec)
+        assert_same(ids, sc, rid, rsc, f"prefix search, {prec}")
+    # a prefix is Euclidean only, and never the low-precision pass
+    for m, prec in (("cos", "fp64"), ("dot", "auto"), ("l2", "bf16")):
+        with pytest.raises(svdb.SvdbError):
+            db7.search(Q[:2, :7], 1, m, prec)
     db7.close()
 
 
