@@ -527,10 +527,16 @@ static hipError_t scan_fp8_t(const LowpScanArgs& a, uint32_t nslices, bool minon
 template <int METRIC>
 static hipError_t scan_fp8_m(const LowpScanArgs& a, uint32_t nslices, bool minonly,
                              hipStream_t s) {
+  // Up to 768 wide: 4 wavefronts of 64 queries (one per SIMD, query
+  // fragments partly in AGPRs).  Every wavefront reads each tile row's
+  // operand from LDS, so twice the queries per wavefront halves LDS traffic
+  // per MFMA, which bounds this scan (768 wide: 27.9 -> 17.8 ms per 5M rows
+  // x 4096 queries against 8 wavefronts of 32).  Wider rows, and the BF16
+  // form, measured faster with 8 wavefronts.
   switch (a.dp) {
-    case 256: return scan_fp8_t<METRIC, 2, 2>(a, nslices, minonly, s);
-    case 512: return scan_fp8_t<METRIC, 4, 2>(a, nslices, minonly, s);
-    case 768: return scan_fp8_t<METRIC, 6, 2>(a, nslices, minonly, s);
+    case 256: return scan_fp8_t<METRIC, 2, 4, false, 4>(a, nslices, minonly, s);
+    case 512: return scan_fp8_t<METRIC, 4, 4, false, 4>(a, nslices, minonly, s);
+    case 768: return scan_fp8_t<METRIC, 6, 4, false, 4>(a, nslices, minonly, s);
     case 1024: return scan_fp8_t<METRIC, 8, 1>(a, nslices, minonly, s);
     case 1536: return scan_fp8_t<METRIC, 12, 1>(a, nslices, minonly, s);
   }
