@@ -682,12 +682,14 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
 }
 
 // Low-precision search of nr metric ranges: range i holds R[i].n queries of
-// metric R[i].metric at query rows R[i].qoff.. (dQ, dids, dscores).  With
-// several ranges (BF16 rows <= 256 wide only) one mixed-metric first pass
-// serves them all: each range takes a run of slots starting on a wavefront
-// boundary (padding slots are flagged and never scanned); the pre-pass,
-// shortlist, rerank, widening and fallback then run per range, each with
-// its own k and shortlist width (k is the largest over ranges).
+// metric R[i].metric at query rows R[i].qoff.. (dQ, dids, dscores).  Several
+// ranges (BF16 rows <= 256 wide only) each take a run of slots starting on a
+// wavefront boundary (padding slots are flagged and never scanned).  The
+// first pass runs one single-metric launch per metric, or one launch of the
+// mixed-metric kernel when all ranges fit one query block; the pre-pass runs
+// per range; the shortlists and the reranks of all ranges share a launch;
+// widening and fallback run per range, each with its own k and shortlist
+// width (k is the largest over ranges).
 int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, int k,
                             uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
   int rc;

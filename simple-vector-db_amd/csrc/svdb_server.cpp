@@ -5,8 +5,9 @@
 // search queued behind them (up to max_batch queries, and while the
 // estimated device memory of the tick stays within the budget) into one
 // batch — any mix of sizes, k and metrics — answered by one call into the
-// engine (one first-pass scan per metric present).  Requests that arrive
-// while a tick is on the GPU wait for the next tick; the queries of the
+// engine (one first pass over the store for the whole tick, by metric).
+// Requests that arrive while a tick is on the GPU wait for the next tick;
+// the queries of the
 // next tick are staged (copied and grouped on the host) by a second thread
 // while the current tick runs, and searches keep joining it until the
 // device thread takes it.  A mutation is a barrier: searches queued
