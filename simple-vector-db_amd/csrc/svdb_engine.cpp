@@ -648,6 +648,7 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
     size_t want = (1024 + groups - 1) / groups;
     size_t maxs = std::max<size_t>(1, nphys_ / 1024);
     nslices = (uint32_t)std::max<size_t>(1, std::min(want, maxs));
+    nslices = (uint32_t)round_up(nslices, 8);  // one set of slices per XCD
     rps = (uint32_t)((nphys_ + nslices - 1) / nslices);
   }
   const uint32_t P = nslices * kWavesPerBlock;

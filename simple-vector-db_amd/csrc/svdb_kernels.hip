@@ -236,8 +236,14 @@ template <int METRIC, int KPL>
 __global__ __launch_bounds__(256) void k_scan_f64(ScanF64Args a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const uint32_t q0 = blockIdx.x * kScanTQ;
-  const uint32_t slice = blockIdx.y;
+  // XCD-aware block order: block ids ≡ x (mod 8) run on XCD x, which gets
+  // whole slices (slice ≡ x mod 8) with all their query groups one after
+  // the other; the groups of a slice then stream its rows through that
+  // XCD's L2 in near lockstep instead of each from HBM.
+  const uint32_t ngroups = (a.nq + kScanTQ - 1) / kScanTQ;
+  const uint32_t xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+  const uint32_t q0 = (local % ngroups) * kScanTQ;
+  const uint32_t slice = (local / ngroups) * 8 + xcd;
   const uint32_t r_begin = slice * a.rows_per_slice;
   const uint32_t r_end = min(r_begin + a.rows_per_slice, a.st.nrows);
   // Queries past nq reuse the last query's row (their lists are not written).
@@ -360,7 +366,9 @@ __global__ __launch_bounds__(256) void k_scan_f64(ScanF64Args a) {
 
 template <int METRIC, int KPL>
 static hipError_t scan_f64_t(const ScanF64Args& a, uint32_t nslices, hipStream_t s) {
-  dim3 grid((a.nq + kScanTQ - 1) / kScanTQ, nslices);
+  // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid
+  if (nslices % 8) return hipErrorInvalidValue;
+  dim3 grid(((a.nq + kScanTQ - 1) / kScanTQ) * nslices);
   hipLaunchKernelGGL((k_scan_f64<METRIC, KPL>), grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
