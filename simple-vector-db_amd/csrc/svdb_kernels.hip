@@ -213,20 +213,27 @@ template <int METRIC>
 __device__ __forceinline__ void accum8(double (&acc)[kScanTQ][2], const double (&x0)[8],
                                        const double (&x1)[8], const double* __restrict__ Qg,
                                        size_t ldq, uint32_t c) {
-  // Query-outer order keeps one query's 8 values live in scalar registers;
-  // each (query, row) chain still advances in element order.
+  // Queries in groups of 4: a group's 32 values sit in scalar registers,
+  // and its 8 (query, row) chains interleave, so each chain's next fma is 8
+  // instructions after its last (FP64 fma latency); every chain still
+  // advances in element order.
+  constexpr int G = 4;
 #pragma unroll
-  for (int j = 0; j < kScanTQ; j++) {
+  for (int j0 = 0; j0 < kScanTQ; j0 += G) {
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      double q = Qg[(size_t)j * ldq + c + i];
-      if (METRIC == kL2) {
-        double t0 = x0[i] - q, t1 = x1[i] - q;
-        acc[j][0] = fma(t0, t0, acc[j][0]);
-        acc[j][1] = fma(t1, t1, acc[j][1]);
-      } else {
-        acc[j][0] = fma(q, x0[i], acc[j][0]);
-        acc[j][1] = fma(q, x1[i], acc[j][1]);
+#pragma unroll
+      for (int jj = 0; jj < G; jj++) {
+        const int j = j0 + jj;
+        double q = Qg[(size_t)j * ldq + c + i];
+        if (METRIC == kL2) {
+          double t0 = x0[i] - q, t1 = x1[i] - q;
+          acc[j][0] = fma(t0, t0, acc[j][0]);
+          acc[j][1] = fma(t1, t1, acc[j][1]);
+        } else {
+          acc[j][0] = fma(q, x0[i], acc[j][0]);
+          acc[j][1] = fma(q, x1[i], acc[j][1]);
+        }
       }
     }
   }
