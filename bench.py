@@ -29,7 +29,7 @@ import numpy as np  # noqa: E402
 METRIC = "queries/s + p50/p99 ms, 10M×128 top-10, ids exact vs FP64 oracle"
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 PEAK_FP64_TFLOPS = 78.6
-PROFILED_TRAFFIC = 2_686_553_472 + 67_108_864  # fetch + write bytes, one k_scan_bf16 launch
+PROFILED_TRAFFIC = 2_686_029_056 + 67_108_864  # fetch + write bytes, one k_scan_bf16 launch
 
 
 def log(*a):
