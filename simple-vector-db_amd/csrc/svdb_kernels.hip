@@ -240,7 +240,8 @@ __device__ __forceinline__ void accum8(double (&acc)[kScanTQ][2], const double (
 }
 
 template <int METRIC, int KPL>
-__global__ __launch_bounds__(256) void k_scan_f64(ScanF64Args a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KPL == 1 ? 3 : 2))) void
+k_scan_f64(ScanF64Args a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   // XCD-aware block order: block ids ≡ x (mod 8) run on XCD x, which gets
