@@ -40,6 +40,7 @@ typedef struct VectorDatabase {
     KDTree* kdtree;         /* find-nearest view of this store */
     pthread_mutex_t mutex;  /* guards the host mirror, as in the reference */
     void* svdb_engine;      /* device store + search engine */
+    void* svdb_uuid;        /* uuid -> position index of the host mirror */
 } VectorDatabase;
 
 /* Replaces vector_db_init (vector_database.c:17-54; vector_database.h:39).

@@ -20,9 +20,11 @@
 #include "kdtree.h"
 #include "svdb.h"
 #include "svdb_engine.h"
+#include "svdb_uuid.h"
 #include "vector_database.h"
 
 using svdb::Engine;
+using svdb::UuidIndex;
 
 static_assert(sizeof(Vector) == 56, "Vector layout must match the reference");
 static_assert(offsetof(Vector, dimension) == 40 && offsetof(Vector, data) == 48, "Vector layout");
@@ -43,6 +45,14 @@ struct TreeIndex {
 };
 
 Engine* eng_of(VectorDatabase* db) { return db ? static_cast<Engine*>(db->svdb_engine) : nullptr; }
+UuidIndex* uuids_of(VectorDatabase* db) {
+  return db ? static_cast<UuidIndex*>(db->svdb_uuid) : nullptr;
+}
+// the index's record of one new row at the end of the mirror
+void index_append(VectorDatabase* db, const Vector& v) {
+  const char* u = v.uuid;
+  uuids_of(db)->append(1, &u, UUID_SIZE);
+}
 
 KDTree* make_tree(size_t dimension, Engine* attached) {
   KDTree* t = (KDTree*)calloc(1, sizeof(KDTree));
@@ -164,9 +174,18 @@ VectorDatabase* vector_db_init(size_t initial_capacity, size_t dimension) {
     return nullptr;
   }
   db->svdb_engine = e;
+  db->svdb_uuid = new (std::nothrow) UuidIndex();
+  if (!db->svdb_uuid) {
+    fprintf(stderr, "Failed to allocate the uuid index\n");
+    delete e;
+    free(db->vectors);
+    free(db);
+    return nullptr;
+  }
   db->kdtree = make_tree(dimension, e);
   if (!db->kdtree) {
     fprintf(stderr, "Failed to create KDTree\n");
+    delete uuids_of(db);
     delete e;
     free(db->vectors);
     free(db);
@@ -175,6 +194,7 @@ VectorDatabase* vector_db_init(size_t initial_capacity, size_t dimension) {
   if (pthread_mutex_init(&db->mutex, nullptr) != 0) {
     fprintf(stderr, "Failed to initialize mutex\n");
     kdtree_free(db->kdtree);
+    delete uuids_of(db);
     delete e;
     free(db->vectors);
     free(db);
@@ -188,6 +208,7 @@ void vector_db_free(VectorDatabase* db) {
   for (size_t i = 0; i < db->size; ++i) free(db->vectors[i].data);
   kdtree_free(db->kdtree);
   delete eng_of(db);
+  delete uuids_of(db);
   free(db->vectors);
   pthread_mutex_destroy(&db->mutex);
   free(db);
@@ -226,6 +247,7 @@ size_t vector_db_insert(VectorDatabase* db, Vector vec) {
     return (size_t)-1;
   }
   db->vectors[db->size] = vec;
+  index_append(db, vec);
   if (db->kdtree) kdtree_insert(db->kdtree, vec.data, db->size);
   size_t index = db->size++;
   pthread_mutex_unlock(&db->mutex);
@@ -240,15 +262,41 @@ Vector* vector_db_read(VectorDatabase* db, size_t index) {
   return v;
 }
 
+// The first row (lowest index) whose uuid equals `uuid` (compared as
+// strncmp over UUID_SIZE), as vector_database.c:145-158 finds by scanning;
+// here an index lookup.  An empty uuid, or an index that disagrees with the
+// mirror (a caller wrote a uuid in place), falls back to the scan, and the
+// latter rebuilds the index.
 Vector* vector_db_read_by_uuid(VectorDatabase* db, const char* uuid) {
   if (!db || !uuid) return nullptr;
   pthread_mutex_lock(&db->mutex);
   Vector* v = nullptr;
-  for (size_t i = 0; i < db->size; ++i)
-    if (strncmp(db->vectors[i].uuid, uuid, UUID_SIZE) == 0) {
-      v = &db->vectors[i];
-      break;
+  UuidIndex* ix = uuids_of(db);
+  const size_t pos = uuid[0] ? ix->find(uuid, UUID_SIZE) : UuidIndex::kNone;
+  if (pos != UuidIndex::kNone && pos < db->size &&
+      strncmp(db->vectors[pos].uuid, uuid, UUID_SIZE) == 0 && ix->size() == db->size) {
+    v = &db->vectors[pos];
+  } else {
+    for (size_t i = 0; i < db->size; ++i)
+      if (strncmp(db->vectors[i].uuid, uuid, UUID_SIZE) == 0) {
+        v = &db->vectors[i];
+        break;
+      }
+    // the index missed a hit, held a stale entry or lost count: rebuild it
+    // from the mirror
+    const bool agreed = pos == UuidIndex::kNone && v == nullptr && !ix->stale() &&
+                        ix->size() == db->size;
+    if (uuid[0] && !agreed) {
+      ix->clear();
+      try {
+        std::vector<const char*> us(db->size);
+        for (size_t i = 0; i < db->size; ++i) us[i] = db->vectors[i].uuid;
+        ix->append(db->size, us.data(), UUID_SIZE);
+      } catch (...) {
+        ix->mark_stale();  // rebuilt by the next lookup
+      }
     }
+  }
   pthread_mutex_unlock(&db->mutex);
   return v;
 }
@@ -258,6 +306,7 @@ void vector_db_update(VectorDatabase* db, size_t index, Vector vec) {
   pthread_mutex_lock(&db->mutex);
   if (index < db->size && vec.data &&
       eng_of(db)->overwrite(index, vec.data, vec.dimension) == SVDB_OK) {
+    uuids_of(db)->rename(index, db->vectors[index].uuid, vec.uuid, UUID_SIZE);
     free(db->vectors[index].data);
     db->vectors[index] = vec;
     if (db->kdtree) kdtree_insert(db->kdtree, vec.data, index);
@@ -287,9 +336,10 @@ int mirror_reserve(VectorDatabase* db, size_t rows) {
 
 void mirror_insert(VectorDatabase* db, Vector vec) {
   pthread_mutex_lock(&db->mutex);
-  if (grow_mirror(db, db->size + 1) == 0)  // reserved by mirror_reserve
+  if (grow_mirror(db, db->size + 1) == 0) {  // reserved by mirror_reserve
+    index_append(db, vec);
     db->vectors[db->size++] = vec;
-  else
+  } else
     free(vec.data);
   pthread_mutex_unlock(&db->mutex);
 }
@@ -309,6 +359,7 @@ void mirror_update(VectorDatabase* db, size_t index, double* data, size_t dim) {
 void mirror_delete(VectorDatabase* db, size_t index) {
   pthread_mutex_lock(&db->mutex);
   if (index < db->size) {
+    uuids_of(db)->erase(index, db->vectors[index].uuid, UUID_SIZE);
     free(db->vectors[index].data);
     memmove(&db->vectors[index], &db->vectors[index + 1],
             (db->size - index - 1) * sizeof(Vector));
@@ -328,6 +379,7 @@ void vector_db_delete(VectorDatabase* db, size_t index) {
     if (eng_of(db)->erase(index) != SVDB_OK) {
       fprintf(stderr, "vector_db_delete: %s\n", svdb::last_error());
     } else {
+      uuids_of(db)->erase(index, db->vectors[index].uuid, UUID_SIZE);
       free(db->vectors[index].data);
       memmove(&db->vectors[index], &db->vectors[index + 1],
               (db->size - index - 1) * sizeof(Vector));
@@ -418,6 +470,7 @@ VectorDatabase* vector_db_load(const char* filename, size_t dimension) {
       fail = true;
       break;
     }
+    index_append(db, v);
     db->vectors[db->size++] = v;
     if (batch.size() >= (size_t)1 << 24) flush();
   }
@@ -571,13 +624,16 @@ int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t di
     return rc;
   }
   if (first_index) *first_index = db->size;
+  std::vector<const char*> us(uuids ? n : 0);
   for (size_t i = 0; i < n; i++) {
     Vector& v = db->vectors[db->size + i];
     memset(v.uuid, 0, UUID_SIZE);
     if (uuids) memcpy(v.uuid, uuids + i * UUID_SIZE, UUID_SIZE);
     v.dimension = dim;
     v.data = copies[i];
+    if (uuids) us[i] = v.uuid;
   }
+  uuids_of(db)->append(n, uuids ? us.data() : nullptr, UUID_SIZE);
   db->size += n;
   pthread_mutex_unlock(&db->mutex);
   return SVDB_OK;

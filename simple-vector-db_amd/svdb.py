@@ -50,7 +50,7 @@ class KDTree(C.Structure):
 class VectorDatabase(C.Structure):
     _fields_ = [("vectors", C.POINTER(Vector)), ("size", C.c_size_t), ("capacity", C.c_size_t),
                 ("kdtree", C.POINTER(KDTree)), ("mutex", C.c_byte * 40),
-                ("svdb_engine", C.c_void_p)]
+                ("svdb_engine", C.c_void_p), ("svdb_uuid", C.c_void_p)]
 
 
 class Stats(C.Structure):

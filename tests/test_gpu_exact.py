@@ -287,3 +287,74 @@ def test_c2_full_store_subset_of_queries(metric):
     rid, rsc = O.topk(X, Q, 10, metric, threads=16)
     assert_same(ids, sc, rid, rsc, metric)
     db.close()
+
+
+# ---------------------------------------------------------------- uuid index
+def test_read_by_uuid_follows_mutations(tmp_path):
+    """vector_db_read_by_uuid answers from an index kept in step with the
+    host mirror: after bulk loads, inserts, updates that change uuids,
+    deletes that shift positions (directly and through the serving loop's
+    mirror thread) and a save/load round trip, it returns the first row with
+    that uuid, as the reference's front-to-back scan does."""
+    rng = np.random.default_rng(31)
+    d = 8
+    pool = [b""] + [f"uuid-{i:03d}".encode() for i in range(20)]
+    counter = [0]
+
+    def row():
+        counter[0] += 1
+        r = rng.standard_normal(d)
+        r[0] = counter[0]  # every row distinct
+        return r
+
+    def pick():
+        return pool[int(rng.integers(len(pool)))]
+
+    def check(db, model, what):
+        for u in pool:
+            hit = next((r for (mu, r) in model if mu == u), None)
+            got = db.read_by_uuid(u)
+            if hit is None:
+                assert got is None, f"{what}: {u!r} found, none expected"
+            else:
+                assert got is not None and got[0] == u and np.array_equal(got[1], hit), \
+                    f"{what}: {u!r} -> wrong row"
+
+    db = svdb.Database(kd_dimension=3)
+    us = [pick() for _ in range(60)]
+    rows = np.stack([row() for _ in range(60)])
+    db.bulk_insert(rows, us)
+    model = list(zip(us, rows))
+    check(db, model, "bulk")
+    for step in range(240):
+        op = int(rng.integers(3))
+        if op == 0 or not model:
+            r, u = row(), pick()
+            db.insert(r, u)
+            model.append((u, r))
+        elif op == 1:
+            i = int(rng.integers(len(model)))
+            db.delete(i)
+            del model[i]
+        else:
+            i = int(rng.integers(len(model)))
+            r, u = row(), pick()
+            db.update(i, r, u)
+            model[i] = (u, r)
+        if step % 8 == 0:
+            check(db, model, f"step {step}")
+    check(db, model, "mutations")
+    # deletes through the serving loop (applied to the mirror by its thread)
+    srv = svdb.Server(db, max_batch=256)
+    for _ in range(5):
+        i = int(rng.integers(len(model)))
+        srv.wait(srv.submit_delete(i))
+        del model[i]
+    srv.stop()
+    check(db, model, "served deletes")
+    path = tmp_path / "store.bin"
+    db.save(path)
+    db.close()
+    db = svdb.Database.load(path, kd_dimension=3)
+    check(db, model, "after load")
+    db.close()
