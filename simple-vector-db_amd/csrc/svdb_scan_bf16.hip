@@ -253,9 +253,12 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
 #pragma unroll
         for (int sub = 0; sub < (active ? kLowpTile / 16 : 0); sub++) {
           const int rl = sub * 16 + rg * 4;
-          const float4 w = *reinterpret_cast<const float4*>(aux + warr * kLowpTile + rl);
+          // this subtile's per-row values, read after its first MFMAs have
+          // issued: they are used one subtile later, and read here, before
+          // the MFMAs, the first MFMA's LDS wait covered their read too
+          // (same-box A/B, Euclidean at 16,384 queries: -2 %)
+          float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
           float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (wcos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
           f32x4 acc[QF];
 #pragma unroll
           for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -265,10 +268,15 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
             for (int f = 0; f < QF; f++)
               acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
             // fragment ks is dead now: its registers take the next subtile's
-            if (sub + 1 < kLowpTile / 16) {
+            if (sub + 1 < kLowpTile / 16 || ks == 0) {
               __builtin_amdgcn_sched_barrier(0);
-              A[ks] = *reinterpret_cast<const bf16x8*>(
-                  cur + G::chunk((sub + 1) * 16 + ql, ks * 4 + rg) * 16);
+              if (sub + 1 < kLowpTile / 16)
+                A[ks] = *reinterpret_cast<const bf16x8*>(
+                    cur + G::chunk((sub + 1) * 16 + ql, ks * 4 + rg) * 16);
+              if (ks == 0) {
+                w = *reinterpret_cast<const float4*>(aux + warr * kLowpTile + rl);
+                if (wcos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+              }
               __builtin_amdgcn_sched_barrier(0);
             }
           }
@@ -297,9 +305,15 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
           for (int ks = 0; ks < KS; ks++)
             A[ks] = *reinterpret_cast<const bf16x8*>(cur + G::chunk(arow, ks * 4 + rg) * 16);
           const int rl = sub * 16 + rg * 4;
-          const float4 w = *reinterpret_cast<const float4*>(aux + warr * kLowpTile + rl);
+          // per-row values after the first MFMAs, as above, at widths <= 128
+          // (at 256 the later read costs registers the kernel does not have)
+          constexpr bool LATE = KS <= 4;
+          float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
           float4 pen = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (wcos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+          if constexpr (!LATE) {
+            w = *reinterpret_cast<const float4*>(aux + warr * kLowpTile + rl);
+            if (wcos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+          }
           f32x4 acc[QF];
 #pragma unroll
           for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -308,6 +322,12 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
 #pragma unroll
             for (int f = 0; f < QF; f++)
               acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks], B[f][ks], acc[f], 0, 0, 0);
+            if (LATE && ks == 0) {
+              __builtin_amdgcn_sched_barrier(0);
+              w = *reinterpret_cast<const float4*>(aux + warr * kLowpTile + rl);
+              if (wcos) pen = *reinterpret_cast<const float4*>(aux + kLowpTile + rl);
+              __builtin_amdgcn_sched_barrier(0);
+            }
           }
           // the previous subtile's epilogue: its common-case test is VALU work
           // in the same basic block as the MFMAs above, free to be scheduled
