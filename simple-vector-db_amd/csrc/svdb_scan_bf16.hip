@@ -288,7 +288,11 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
             const float pv[4] = {ppen.x, ppen.y, ppen.z, ppen.w};
             bool need = true;
             if constexpr (!MINONLY && !MIXED) need = need_of(pacc, wv, pv);
-            if (need) insert_all(pacc, wv, pv, prow);
+            // a wave-uniform, unlikely branch: the insertion code is laid out
+            // away from the loop and the common path falls through to the
+            // next subtile with no exec-mask changes (same-box A/B: -0.7 %
+            // Euclidean, -1.1 % cosine)
+            if (__builtin_expect(__ballot(need) != 0, 0)) insert_all(pacc, wv, pv, prow);
           }
 #pragma unroll
           for (int f = 0; f < QF; f++) pacc[f] = acc[f];
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(512) void k_scan_bf16(LowpScanArgs a) {
             const float pv[4] = {ppen.x, ppen.y, ppen.z, ppen.w};
             bool need = true;
             if constexpr (!MINONLY && !MIXED) need = need_of(pacc, wv, pv);
-            if (need) insert_all(pacc, wv, pv, prow);
+            if (__builtin_expect(__ballot(need) != 0, 0)) insert_all(pacc, wv, pv, prow);
           }
 #pragma unroll
           for (int f = 0; f < QF; f++) pacc[f] = acc[f];
