@@ -411,8 +411,8 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
+              "(row or query sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     if args.wide:
