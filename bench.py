@@ -70,8 +70,12 @@ def fp64_c2(args):
                     np.array_equal(sc[:64].view(np.uint64), rsc.view(np.uint64)))
         scan_ms = kt[1] / max(1, kt[0])
         tf = 2.0 * X.shape[0] * 128 * B / (scan_ms * 1e-3) / 1e12
+        # algorithmic HBM bytes of one launch: the FP64 rows (and the row
+        # norms for cosine) once, and the batch's queries
+        nbytes = X.shape[0] * (128 + (metric == "cos")) * 8 + B * 128 * 8
         out[metric] = {"qps": round(10 * B / el, 1), "scan_ms_per_batch": round(scan_ms, 3),
                        "tflops": round(tf, 2), "frac_of_fp64_peak": round(tf / PEAK_FP64_TFLOPS, 4),
+                       "hbm_gbs_algorithmic": round(nbytes / (scan_ms * 1e-3) / 1e9, 1),
                        "ids_scores_equal_cpu_oracle_64": same}
         log(f"fp64 scan C2 {metric}: {out[metric]}")
     for a in (dQ, dids, dsc):

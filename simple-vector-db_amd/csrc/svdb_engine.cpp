@@ -644,8 +644,14 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
   uint32_t nslices = 0, rps = 0;
   if (nphys_ > 0) {
+    // About 8,192 blocks (query groups × row slices): short slices keep the
+    // query groups of one slice, which run back to back on one XCD, close
+    // enough together for that XCD's L2 to serve them all the slice's rows
+    // (1M × 128, 1,024 queries: 15.6 → 12.7 ms Euclidean, 12.0 → 9.9 ms dot
+    // against 1,024 blocks; 16,384 lost again in the merge).  At most 1,024
+    // slices, so a few queries do not leave the merge thousands of lists.
     const size_t groups = (nq + kScanTQ - 1) / kScanTQ;
-    size_t want = (1024 + groups - 1) / groups;
+    size_t want = std::min<size_t>(1024, (8192 + groups - 1) / groups);
     size_t maxs = std::max<size_t>(1, nphys_ / 1024);
     nslices = (uint32_t)std::max<size_t>(1, std::min(want, maxs));
     nslices = (uint32_t)round_up(nslices, 8);  // one set of slices per XCD
