@@ -264,8 +264,10 @@ Vector* vector_db_read(VectorDatabase* db, size_t index) {
 
 // The first row (lowest index) whose uuid equals `uuid` (compared as
 // strncmp over UUID_SIZE), as vector_database.c:145-158 finds by scanning;
-// here an index lookup.  An empty uuid, or an index that disagrees with the
-// mirror (a caller wrote a uuid in place), falls back to the scan, and the
+// here an index lookup, exact as long as uuids change only through these
+// entry points.  An empty uuid, or an index caught disagreeing with the
+// mirror (its row no longer holds the uuid, its count is off, or it is
+// stale after an allocation failure), falls back to the scan, and the
 // latter rebuilds the index.
 Vector* vector_db_read_by_uuid(VectorDatabase* db, const char* uuid) {
   if (!db || !uuid) return nullptr;
