@@ -654,6 +654,11 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
     size_t want = std::min<size_t>(1024, (8192 + groups - 1) / groups);
     size_t maxs = std::max<size_t>(1, nphys_ / 1024);
     nslices = (uint32_t)std::max<size_t>(1, std::min(want, maxs));
+    // partial lists (nq × slices × waves × kmax entries of 12 bytes) stay
+    // under 2^25 entries (~400 MB): large batches at large k take fewer,
+    // longer slices
+    while (nslices > 8 && (size_t)nq * nslices * kWavesPerBlock * kmax > ((size_t)1 << 25))
+      nslices = (nslices + 1) / 2;
     nslices = (uint32_t)round_up(nslices, 8);  // one set of slices per XCD
     rps = (uint32_t)((nphys_ + nslices - 1) / nslices);
   }
