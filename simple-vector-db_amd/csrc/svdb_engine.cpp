@@ -641,6 +641,7 @@ static constexpr size_t kMaxBatchLowp = 16384;
 int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk,
                       int metric, int kmax, uint64_t* dids, double* dscores, uint32_t ostride) {
   int rc;
+  if (nq == 0) return SVDB_OK;
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
   uint32_t nslices = 0, rps = 0;
   if (nphys_ > 0) {
