@@ -106,13 +106,22 @@ void kdtree_insert(KDTree* tree, const double* point, size_t index) {
       fprintf(stderr, "kdtree_insert: %s\n", svdb::last_error());
     return;
   }
+  // label bookkeeping is allocated before the row is appended, so an
+  // allocation failure leaves the tree unchanged
+  try {
+    ix->labels.reserve(ix->labels.size() + 1);
+    ix->by_label.emplace(index, ix->labels.size());
+  } catch (...) {
+    fprintf(stderr, "kdtree_insert: out of host memory\n");
+    return;
+  }
   uint32_t slot = 0;
   if (ix->eng->append(point, 1, tree->dimension, tree->dimension, &slot) != SVDB_OK) {
     fprintf(stderr, "kdtree_insert: %s\n", svdb::last_error());
+    ix->by_label.erase(index);
     return;
   }
-  ix->by_label[index] = ix->labels.size();
-  ix->labels.push_back(index);
+  ix->labels.push_back(index);  // capacity reserved above
 }
 
 void kdtree_free(KDTree* tree) {
@@ -466,7 +475,14 @@ VectorDatabase* vector_db_load(const char* filename, size_t dimension) {
     }
     if (batch_dim != v.dimension && !batch.empty()) flush();
     batch_dim = v.dimension;
-    batch.insert(batch.end(), v.data, v.data + v.dimension);
+    try {
+      batch.insert(batch.end(), v.data, v.data + v.dimension);
+    } catch (...) {
+      fprintf(stderr, "vector_db_load: out of host memory at record %zu\n", i);
+      free(v.data);
+      fail = true;
+      break;
+    }
     if (grow_mirror(db, db->size + 1) != 0) {
       free(v.data);
       fail = true;
@@ -608,7 +624,16 @@ int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t di
     return SVDB_ENOMEM;
   }
   // Host copies first, so a failure leaves the store unchanged.
-  std::vector<double*> copies(n, nullptr);
+  std::vector<double*> copies;
+  std::vector<const char*> us;
+  try {
+    copies.assign(n, nullptr);
+    us.assign(uuids ? n : 0, nullptr);
+  } catch (...) {
+    pthread_mutex_unlock(&db->mutex);
+    svdb::set_error(SVDB_ENOMEM, "host allocation failed");
+    return SVDB_ENOMEM;
+  }
   for (size_t i = 0; i < n; i++) {
     copies[i] = (double*)malloc(dim * sizeof(double));
     if (!copies[i]) {
@@ -626,7 +651,6 @@ int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t di
     return rc;
   }
   if (first_index) *first_index = db->size;
-  std::vector<const char*> us(uuids ? n : 0);
   for (size_t i = 0; i < n; i++) {
     Vector& v = db->vectors[db->size + i];
     memset(v.uuid, 0, UUID_SIZE);
