@@ -48,6 +48,28 @@ Engine* eng_of(VectorDatabase* db) { return db ? static_cast<Engine*>(db->svdb_e
 UuidIndex* uuids_of(VectorDatabase* db) {
   return db ? static_cast<UuidIndex*>(db->svdb_uuid) : nullptr;
 }
+// Runs an engine call from a C entry point: a host allocation failure is
+// reported as SVDB_ENOMEM (any other exception as SVDB_EINVAL), and no
+// exception crosses the C boundary (the engine's own lock is released by
+// its guard on the way out).
+template <class F>
+int c_call(F&& f) {
+  int code = SVDB_EINVAL;
+  const char* msg = "internal error";
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    code = SVDB_ENOMEM;
+    msg = "host allocation failed";
+  } catch (...) {
+  }
+  try {
+    svdb::set_error(code, msg);
+  } catch (...) {
+  }
+  return code;
+}
+
 // the index's record of one new row at the end of the mirror
 void index_append(VectorDatabase* db, const Vector& v) {
   const char* u = v.uuid;
@@ -587,7 +609,10 @@ int svdb_search(VectorDatabase* db, const double* queries, size_t nq, size_t dim
     svdb::set_error(SVDB_EINVAL, "query dimension exceeds the store's");
     return SVDB_EINVAL;
   }
-  return e->search_host(queries, nq, dim, dim, k, metric, precision, out_stride, ids_out, scores_out);
+  return c_call([&] {
+    return e->search_host(queries, nq, dim, dim, k, metric, precision, out_stride, ids_out,
+                          scores_out);
+  });
 }
 
 int svdb_search_device(VectorDatabase* db, const double* d_queries, size_t nq, size_t ldq, size_t k,
@@ -597,7 +622,9 @@ int svdb_search_device(VectorDatabase* db, const double* d_queries, size_t nq, s
     svdb::set_error(SVDB_EINVAL, "NULL argument");
     return SVDB_EINVAL;
   }
-  return e->search_device(d_queries, nq, ldq, e->dim(), k, metric, precision, d_ids, d_scores);
+  return c_call([&] {
+    return e->search_device(d_queries, nq, ldq, e->dim(), k, metric, precision, d_ids, d_scores);
+  });
 }
 
 int svdb_pair_metrics(VectorDatabase* db, const size_t* index_a, const size_t* index_b,
@@ -607,7 +634,7 @@ int svdb_pair_metrics(VectorDatabase* db, const size_t* index_a, const size_t* i
     svdb::set_error(SVDB_EINVAL, "NULL argument");
     return SVDB_EINVAL;
   }
-  return e->pairs_f64(index_a, index_b, npairs, metric, out);
+  return c_call([&] { return e->pairs_f64(index_a, index_b, npairs, metric, out); });
 }
 
 int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t dim,

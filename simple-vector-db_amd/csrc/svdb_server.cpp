@@ -19,6 +19,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <new>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -101,8 +102,13 @@ class Server {
     if (stop_) return SVDB_EINVAL;
     r->ticket = next_++;
     r->t_submit = now_us();
-    all_[r->ticket] = r;
-    queue_.push_back(r);
+    all_[r->ticket] = r;  // may throw (host memory): nothing queued then
+    try {
+      queue_.push_back(r);
+    } catch (...) {
+      all_.erase(r->ticket);
+      throw;
+    }
     cv_q_.notify_all();
     return r->ticket;
   }
@@ -185,10 +191,18 @@ class Server {
             break;
           queue_.pop_front();
           const size_t o = t->nq;
-          t->q.resize((o + r->nq) * d);
+          try {  // room first: a host allocation failure fails this request only
+            t->q.resize((o + r->nq) * d);
+            t->k.resize(o + r->nq, r->k);
+            t->metric.resize(o + r->nq, r->metric);
+            t->reqs.reserve(t->reqs.size() + 1);
+          } catch (const std::bad_alloc&) {
+            r->rc = SVDB_ENOMEM;
+            r->done = true;
+            cv_done_.notify_all();
+            continue;
+          }
           std::memcpy(&t->q[o * d], r->data.data(), r->nq * d * sizeof(double));
-          t->k.resize(o + r->nq, r->k);
-          t->metric.resize(o + r->nq, r->metric);
           t->kmax = std::max(t->kmax, r->k);
           t->nq += r->nq;
           t->bytes += r->est_bytes;
@@ -231,7 +245,7 @@ class Server {
         for (auto& m : t->mutations) {
           run_mutation(*m);
           if (m->rc != SVDB_OK) {
-            finish({m}, t0, tick, 0);
+            finish(m, t0, tick, 0);
           } else {
             std::lock_guard<std::mutex> g(mu_);
             m->t_start = t0;
@@ -243,18 +257,31 @@ class Server {
         continue;
       }
       const size_t d = eng_->dim();
-      std::vector<size_t> ids(t->nq * t->kmax);
-      std::vector<double> sc(t->nq * t->kmax);
+      std::vector<size_t> ids;
+      std::vector<double> sc;
       int rc = SVDB_OK;
-      if (t->nq)
-        rc = eng_->search_host(t->q.data(), t->nq, d, d, t->k.data(), t->metric.data(),
-                               cfg_.precision, t->kmax, ids.data(), sc.data());
+      try {  // a host allocation failure fails the tick's searches
+        ids.resize(t->nq * t->kmax);
+        sc.resize(t->nq * t->kmax);
+        if (t->nq)
+          rc = eng_->search_host(t->q.data(), t->nq, d, d, t->k.data(), t->metric.data(),
+                                 cfg_.precision, t->kmax, ids.data(), sc.data());
+      } catch (const std::bad_alloc&) {
+        set_error(SVDB_ENOMEM, "host allocation failed");
+        rc = SVDB_ENOMEM;
+      }
       size_t o = 0;
       for (auto& r : t->reqs) {
         r->rc = rc;
         if (rc == SVDB_OK) {
-          r->ids.resize(r->nq * r->k);
-          r->scores.resize(r->nq * r->k);
+          try {
+            r->ids.resize(r->nq * r->k);
+            r->scores.resize(r->nq * r->k);
+          } catch (const std::bad_alloc&) {
+            r->rc = SVDB_ENOMEM;
+            o += r->nq;
+            continue;
+          }
           for (size_t i = 0; i < r->nq; i++)
             for (size_t j = 0; j < r->k; j++) {
               r->ids[i * r->k + j] = ids[(o + i) * t->kmax + j];
@@ -312,20 +339,30 @@ class Server {
       } else {
         mirror_delete(db_, r->index);
       }
-      finish({r}, r->t_start, r->tick, 0);
+      finish(r, r->t_start, r->tick, 0);
     }
   }
 
-  void finish(std::vector<std::shared_ptr<Request>> rs, int64_t t0, uint64_t tick, size_t nq) {
+  // Completes requests (allocates nothing).
+  void finish(const std::vector<std::shared_ptr<Request>>& rs, int64_t t0, uint64_t tick,
+              size_t nq) {
+    finish(rs.data(), rs.size(), t0, tick, nq);
+  }
+  void finish(const std::shared_ptr<Request>& r, int64_t t0, uint64_t tick, size_t nq) {
+    finish(&r, 1, t0, tick, nq);
+  }
+  void finish(const std::shared_ptr<Request>* rs, size_t n, int64_t t0, uint64_t tick,
+              size_t nq) {
     const int64_t t1 = now_us();
     {
       std::lock_guard<std::mutex> g(mu_);
-      for (auto& r : rs) {
-        r->t_start = t0;
-        r->t_done = t1;
-        r->tick = tick;
-        r->tick_queries = nq;
-        r->done = true;
+      for (size_t i = 0; i < n; i++) {
+        Request& r = *rs[i];
+        r.t_start = t0;
+        r.t_done = t1;
+        r.tick = tick;
+        r.tick_queries = nq;
+        r.done = true;
       }
     }
     cv_done_.notify_all();
@@ -352,6 +389,28 @@ class Server {
 using svdb::Request;
 using svdb::Server;
 
+namespace {
+// The C entry points report a host allocation failure as SVDB_ENOMEM (any
+// other exception as SVDB_EINVAL) and let no exception cross the C boundary.
+template <class F>
+int64_t c_call(F&& f) {
+  int code = SVDB_EINVAL;
+  const char* msg = "internal error";
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    code = SVDB_ENOMEM;
+    msg = "host allocation failed";
+  } catch (...) {
+  }
+  try {
+    svdb::set_error(code, msg);
+  } catch (...) {
+  }
+  return code;
+}
+}  // namespace
+
 extern "C" {
 
 void svdb_server_default_config(svdb_server_config* cfg) {
@@ -373,7 +432,13 @@ svdb_server* svdb_server_start(VectorDatabase* db, const svdb_server_config* cfg
     svdb::set_error(SVDB_EINVAL, "max_batch and budget_bytes must be positive");
     return nullptr;
   }
-  return reinterpret_cast<svdb_server*>(new Server(db, c));
+  Server* sv = nullptr;
+  if (c_call([&]() -> int64_t {
+        sv = new Server(db, c);
+        return SVDB_OK;
+      }) != SVDB_OK)
+    return nullptr;
+  return reinterpret_cast<svdb_server*>(sv);
 }
 
 void svdb_server_stop(svdb_server* s) { delete reinterpret_cast<Server*>(s); }
@@ -385,23 +450,25 @@ int64_t svdb_server_submit_search(svdb_server* s, const double* queries, size_t 
     svdb::set_error(SVDB_EINVAL, "bad search request");
     return SVDB_EINVAL;
   }
-  auto r = std::make_shared<Request>();
-  r->kind = 0;
-  r->nq = nq;
-  r->k = k;
-  r->metric = metric;
-  r->dim = dim;
-  if (sv->dim() != 0 && dim != sv->dim()) {
-    svdb::set_error(SVDB_EINVAL, "query dimension differs from the store's");
-    return SVDB_EINVAL;
-  }
-  r->est_bytes = sv->query_bytes(nq, k);
-  if (r->est_bytes > sv->cfg().budget_bytes || nq > sv->cfg().max_batch) {
-    svdb::set_error(SVDB_ENOMEM, "request larger than the server's batch or memory budget");
-    return SVDB_ENOMEM;
-  }
-  r->data.assign(queries, queries + nq * dim);
-  return sv->submit(r);
+  return c_call([&]() -> int64_t {
+    auto r = std::make_shared<Request>();
+    r->kind = 0;
+    r->nq = nq;
+    r->k = k;
+    r->metric = metric;
+    r->dim = dim;
+    if (sv->dim() != 0 && dim != sv->dim()) {
+      svdb::set_error(SVDB_EINVAL, "query dimension differs from the store's");
+      return SVDB_EINVAL;
+    }
+    r->est_bytes = sv->query_bytes(nq, k);
+    if (r->est_bytes > sv->cfg().budget_bytes || nq > sv->cfg().max_batch) {
+      svdb::set_error(SVDB_ENOMEM, "request larger than the server's batch or memory budget");
+      return SVDB_ENOMEM;
+    }
+    r->data.assign(queries, queries + nq * dim);
+    return sv->submit(r);
+  });
 }
 
 int64_t svdb_server_submit_insert(svdb_server* s, const double* row, size_t dim) {
@@ -410,11 +477,13 @@ int64_t svdb_server_submit_insert(svdb_server* s, const double* row, size_t dim)
     svdb::set_error(SVDB_EINVAL, "bad insert request");
     return SVDB_EINVAL;
   }
-  auto r = std::make_shared<Request>();
-  r->kind = 1;
-  r->dim = dim;
-  r->data.assign(row, row + dim);
-  return sv->submit(r);
+  return c_call([&]() -> int64_t {
+    auto r = std::make_shared<Request>();
+    r->kind = 1;
+    r->dim = dim;
+    r->data.assign(row, row + dim);
+    return sv->submit(r);
+  });
 }
 
 int64_t svdb_server_submit_update(svdb_server* s, size_t index, const double* row, size_t dim) {
@@ -423,12 +492,14 @@ int64_t svdb_server_submit_update(svdb_server* s, size_t index, const double* ro
     svdb::set_error(SVDB_EINVAL, "bad update request");
     return SVDB_EINVAL;
   }
-  auto r = std::make_shared<Request>();
-  r->kind = 2;
-  r->index = index;
-  r->dim = dim;
-  r->data.assign(row, row + dim);
-  return sv->submit(r);
+  return c_call([&]() -> int64_t {
+    auto r = std::make_shared<Request>();
+    r->kind = 2;
+    r->index = index;
+    r->dim = dim;
+    r->data.assign(row, row + dim);
+    return sv->submit(r);
+  });
 }
 
 int64_t svdb_server_submit_delete(svdb_server* s, size_t index) {
@@ -437,10 +508,12 @@ int64_t svdb_server_submit_delete(svdb_server* s, size_t index) {
     svdb::set_error(SVDB_EINVAL, "NULL server");
     return SVDB_EINVAL;
   }
-  auto r = std::make_shared<Request>();
-  r->kind = 3;
-  r->index = index;
-  return sv->submit(r);
+  return c_call([&]() -> int64_t {
+    auto r = std::make_shared<Request>();
+    r->kind = 3;
+    r->index = index;
+    return sv->submit(r);
+  });
 }
 
 int svdb_server_wait(svdb_server* s, int64_t ticket, size_t* ids, double* scores,
@@ -450,7 +523,7 @@ int svdb_server_wait(svdb_server* s, int64_t ticket, size_t* ids, double* scores
     svdb::set_error(SVDB_EINVAL, "NULL server");
     return SVDB_EINVAL;
   }
-  return sv->wait(ticket, ids, scores, info);
+  return (int)c_call([&]() -> int64_t { return sv->wait(ticket, ids, scores, info); });
 }
 
 }  // extern "C"
