@@ -124,7 +124,7 @@ void kdtree_insert(KDTree* tree, const double* point, size_t index) {
   if (!ix->owned) return;  // the database store already holds the row
   auto it = ix->by_label.find(index);
   if (it != ix->by_label.end()) {
-    if (ix->eng->overwrite(it->second, point, tree->dimension) != SVDB_OK)
+    if (c_call([&] { return ix->eng->overwrite(it->second, point, tree->dimension); }) != SVDB_OK)
       fprintf(stderr, "kdtree_insert: %s\n", svdb::last_error());
     return;
   }
@@ -138,7 +138,8 @@ void kdtree_insert(KDTree* tree, const double* point, size_t index) {
     return;
   }
   uint32_t slot = 0;
-  if (ix->eng->append(point, 1, tree->dimension, tree->dimension, &slot) != SVDB_OK) {
+  if (c_call([&] { return ix->eng->append(point, 1, tree->dimension, tree->dimension, &slot); }) !=
+      SVDB_OK) {
     fprintf(stderr, "kdtree_insert: %s\n", svdb::last_error());
     ix->by_label.erase(index);
     return;
@@ -170,7 +171,8 @@ size_t kdtree_nearest(KDTree* tree, const double* point) {
   int metric = SVDB_METRIC_L2;
   double score;
   const int prec = D == e->dim() ? SVDB_PREC_AUTO : SVDB_PREC_FP64;
-  if (e->search_host(point, 1, D, D, &k, &metric, prec, 1, &id, &score) != SVDB_OK) {
+  if (c_call([&] { return e->search_host(point, 1, D, D, &k, &metric, prec, 1, &id, &score); }) !=
+      SVDB_OK) {
     fprintf(stderr, "kdtree_nearest: %s\n", svdb::last_error());
     return (size_t)-1;
   }
@@ -272,7 +274,9 @@ size_t vector_db_insert(VectorDatabase* db, Vector vec) {
     pthread_mutex_unlock(&db->mutex);
     return (size_t)-1;
   }
-  if (!vec.data || eng_of(db)->append(vec.data, 1, vec.dimension, vec.dimension, nullptr) != SVDB_OK) {
+  if (!vec.data || c_call([&] {
+        return eng_of(db)->append(vec.data, 1, vec.dimension, vec.dimension, nullptr);
+      }) != SVDB_OK) {
     fprintf(stderr, "vector_db_insert: %s\n", vec.data ? svdb::last_error() : "NULL data");
     pthread_mutex_unlock(&db->mutex);
     return (size_t)-1;
@@ -338,7 +342,7 @@ void vector_db_update(VectorDatabase* db, size_t index, Vector vec) {
   if (!db) return;
   pthread_mutex_lock(&db->mutex);
   if (index < db->size && vec.data &&
-      eng_of(db)->overwrite(index, vec.data, vec.dimension) == SVDB_OK) {
+      c_call([&] { return eng_of(db)->overwrite(index, vec.data, vec.dimension); }) == SVDB_OK) {
     uuids_of(db)->rename(index, db->vectors[index].uuid, vec.uuid, UUID_SIZE);
     free(db->vectors[index].data);
     db->vectors[index] = vec;
@@ -409,7 +413,7 @@ void vector_db_delete(VectorDatabase* db, size_t index) {
   if (!db) return;
   pthread_mutex_lock(&db->mutex);
   if (index < db->size) {
-    if (eng_of(db)->erase(index) != SVDB_OK) {
+    if (c_call([&] { return eng_of(db)->erase(index); }) != SVDB_OK) {
       fprintf(stderr, "vector_db_delete: %s\n", svdb::last_error());
     } else {
       uuids_of(db)->erase(index, db->vectors[index].uuid, UUID_SIZE);
@@ -473,7 +477,9 @@ VectorDatabase* vector_db_load(const char* filename, size_t dimension) {
   bool fail = false;
   auto flush = [&]() {
     size_t rows = batch_dim ? batch.size() / batch_dim : 0;
-    if (rows && eng_of(db)->append(batch.data(), rows, batch_dim, batch_dim, nullptr) != SVDB_OK) {
+    if (rows && c_call([&] {
+          return eng_of(db)->append(batch.data(), rows, batch_dim, batch_dim, nullptr);
+        }) != SVDB_OK) {
       fprintf(stderr, "vector_db_load: %s\n", svdb::last_error());
       fail = true;
     }
@@ -671,7 +677,7 @@ int svdb_bulk_insert(VectorDatabase* db, const double* rows, size_t n, size_t di
     }
     memcpy(copies[i], rows + i * dim, dim * sizeof(double));
   }
-  int rc = e->append(rows, n, dim, dim, nullptr);
+  int rc = c_call([&] { return e->append(rows, n, dim, dim, nullptr); });
   if (rc != SVDB_OK) {
     for (double* p : copies) free(p);
     pthread_mutex_unlock(&db->mutex);
@@ -707,7 +713,7 @@ int svdb_set_budget_bytes(VectorDatabase* db, size_t bytes) {
 int svdb_enable_shadow(VectorDatabase* db, int precision) {
   Engine* e = eng_of(db);
   if (!e) return SVDB_EINVAL;
-  return e->enable_shadow(precision);
+  return c_call([&] { return e->enable_shadow(precision); });
 }
 
 int svdb_get_stats(VectorDatabase* db, svdb_stats* out) {
@@ -732,7 +738,7 @@ int svdb_get_stats(VectorDatabase* db, svdb_stats* out) {
 int svdb_set_profiling(VectorDatabase* db, int on) {
   Engine* e = eng_of(db);
   if (!e) return SVDB_EINVAL;
-  return e->set_profiling(on != 0);
+  return c_call([&] { return e->set_profiling(on != 0); });
 }
 
 int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out) {
