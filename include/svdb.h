@@ -110,6 +110,8 @@ typedef struct svdb_stats {
     double last_max_rel_err;/* max |first-pass score - FP64 score| / |FP64| of the last call */
     uint64_t widened;       /* of the certified: certified only by a wider retry (1024, 4096) */
     uint64_t second_pass;   /* FP8 queries handed to the BF16 first pass (counted again there) */
+    uint64_t compactions;   /* tombstone compactions (live rows moved to the front, shadows
+                               rebuilt) */
 } svdb_stats;
 int svdb_get_stats(VectorDatabase* db, svdb_stats* out);
 
@@ -177,6 +179,13 @@ int svdb_device_copy(void* dst, const void* src, size_t bytes, int kind);
 int svdb_device_sync(void);
 
 const char* svdb_last_error(void);
+
+/* Test hook: the n-th host allocation point reached from now on (in any
+ * thread) fails as an out-of-memory would (std::bad_alloc inside the
+ * library, reported as SVDB_ENOMEM or the reference's sentinel); n = 0
+ * disarms.  Returns the number of points still to pass before the armed
+ * failure (0 once it has fired or when disarmed). */
+int64_t svdb_debug_fail_host_alloc(int64_t n);
 
 #ifdef __cplusplus
 }

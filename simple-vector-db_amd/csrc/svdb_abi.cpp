@@ -12,6 +12,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <unordered_map>
@@ -192,6 +193,11 @@ VectorDatabase* vector_db_init(size_t initial_capacity, size_t dimension) {
   }
   db->size = 0;
   db->capacity = initial_capacity > 0 ? initial_capacity : 10;
+  if (db->capacity > SIZE_MAX / sizeof(Vector)) {
+    fprintf(stderr, "Capacity overflow detected, unable to allocate memory for vectors\n");
+    free(db);
+    return nullptr;
+  }
   db->vectors = (Vector*)malloc(db->capacity * sizeof(Vector));
   if (!db->vectors) {
     fprintf(stderr, "Failed to allocate memory for vectors\n");
@@ -465,7 +471,21 @@ VectorDatabase* vector_db_load(const char* filename, size_t dimension) {
     fclose(f);
     return nullptr;
   }
-  VectorDatabase* db = vector_db_init(n > 0 ? n : 10, dimension);
+  // The header's count sizes the store only as far as the file can hold
+  // that many records (each at least uuid[37] + size_t): a damaged or
+  // crafted count must not become a huge allocation.
+  size_t cap = n;
+  long here = ftell(f);
+  if (here >= 0 && fseek(f, 0, SEEK_END) == 0) {
+    const long end = ftell(f);
+    if (end >= here) cap = std::min(cap, (size_t)(end - here) / (UUID_SIZE + sizeof(size_t)));
+    if (fseek(f, here, SEEK_SET) != 0) {
+      perror("Failed to read file");
+      fclose(f);
+      return nullptr;
+    }
+  }
+  VectorDatabase* db = vector_db_init(cap > 0 ? cap : 10, dimension);
   if (!db) {
     fclose(f);
     return nullptr;
@@ -551,6 +571,11 @@ float dot_product(Vector vec1, Vector vec2) { return legacy_metric(vec1, vec2, S
 // ======================================================================
 
 const char* svdb_last_error(void) { return svdb::last_error(); }
+
+int64_t svdb_debug_fail_host_alloc(int64_t n) {
+  if (n >= 0) svdb::arm_fault(n);
+  return svdb::fault_armed();
+}
 
 void* svdb_device_alloc(size_t bytes) {
   if (svdb::device_init() < 0) return nullptr;
@@ -732,6 +757,7 @@ int svdb_get_stats(VectorDatabase* db, svdb_stats* out) {
   out->last_max_rel_err = s.last_max_rel_err;
   out->widened = s.widened;
   out->second_pass = s.second_pass;
+  out->compactions = s.compactions;
   return SVDB_OK;
 }
 

@@ -2,10 +2,12 @@
 #include "svdb_engine.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 
 namespace svdb {
 
@@ -19,6 +21,24 @@ void set_error(int code, const std::string& msg) {
 }
 
 const char* last_error() { return g_err.c_str(); }
+
+// Countdown of the armed fault: 0 = disarmed; n > 0 = the n-th fault_point
+// from now throws.  Process-wide.
+static std::atomic<int64_t> g_fault_countdown{0};
+
+void fault_point() {
+  int64_t c = g_fault_countdown.load(std::memory_order_relaxed);
+  while (c > 0) {
+    if (g_fault_countdown.compare_exchange_weak(c, c - 1, std::memory_order_relaxed)) {
+      if (c == 1) throw std::bad_alloc();
+      return;
+    }
+  }
+}
+
+void arm_fault(int64_t n) { g_fault_countdown.store(n > 0 ? n : 0); }
+
+int64_t fault_armed() { return g_fault_countdown.load(); }
 
 #define HIPCHK(expr, what)                                                            \
   do {                                                                                \
@@ -234,6 +254,12 @@ int Engine::ensure_capacity(size_t need) {
   }
   size_t nc = round_up(std::max({need, cap_ * 2, init_slots_}), 128);
   nc = std::min(nc, (size_t)UINT32_MAX / 128 * 128 - 128);
+  const size_t nw = nc / 64, ow = cap_ / 64;
+  // Host bitmap first: if its growth throws, nothing has changed yet (cap_
+  // never runs ahead of h_alive_ / h_rank_, which append indexes by slot).
+  fault_point();
+  h_alive_.resize(nw, 0);
+  h_rank_.resize(nw, 0);
   if (lowp_) {
     int rs = alloc_shadow(nc);
     if (rs) return rs;
@@ -242,7 +268,6 @@ int Engine::ensure_capacity(size_t need) {
     int rs = alloc_shadow8(nc);
     if (rs) return rs;
   }
-  const size_t nw = nc / 64, ow = cap_ / 64;
   double *X = nullptr, *nx2 = nullptr, *xn = nullptr;
   uint64_t* al = nullptr;
   uint32_t* rk = nullptr;
@@ -275,8 +300,6 @@ int Engine::ensure_capacity(size_t need) {
   alive_ = al;
   rank_ = rk;
   cap_ = nc;
-  h_alive_.resize(nw, 0);
-  h_rank_.resize(nw, 0);
   dirty_word_ = 0;  // the new bitmap arrays hold nothing yet
   return SVDB_OK;
 }
@@ -441,6 +464,21 @@ int Engine::sync_alive() {
   return SVDB_OK;
 }
 
+// After deletes or rewrites, the shadows' degenerate lists are rebuilt
+// from the live slots before the next low-precision search (call after
+// sync_alive: the kernel reads the device bitmap).
+int Engine::refresh_degen() {
+  if (!degen_dirty_) return SVDB_OK;
+  if (lowp_)
+    HIPCHK(launch_rebuild_degen(sh_.pen, alive_, (uint32_t)nphys_, sh_.degen_count, sh_.degen,
+                                stream_), "degenerate list");
+  if (lowp8_)
+    HIPCHK(launch_rebuild_degen(sh8_.pen, alive_, (uint32_t)nphys_, sh8_.degen_count, sh8_.degen,
+                                stream_), "degenerate list");
+  degen_dirty_ = false;
+  return SVDB_OK;
+}
+
 uint32_t Engine::select_slot(size_t logical) const {
   if (!has_deletes_) return (uint32_t)logical;
   const size_t nw = (nphys_ + 63) / 64;
@@ -519,6 +557,8 @@ int Engine::overwrite(size_t logical, const double* row, size_t dim) {
     return SVDB_EINVAL;
   }
   const uint32_t s = select_slot(logical);
+  // a rewritten row may leave (or join) the degenerate list
+  degen_dirty_ = lowp_ || lowp8_;
   HIPCHK(hipMemcpyAsync(X_ + (size_t)s * ld_, row, d_ * sizeof(double), hipMemcpyHostToDevice,
                         stream_), "row upload");
   HIPCHK(launch_row_norms(X_, ld_, nullptr, s, 1, (uint32_t)d_, nx2_, xn_, stream_), "k_row_norms");
@@ -553,6 +593,105 @@ int Engine::erase(size_t logical) {
   nalive_--;
   has_deletes_ = true;
   dirty_word_ = std::min(dirty_word_, w);
+  degen_dirty_ = lowp_ || lowp8_;
+  // Tombstones past a quarter of the slots (and at least kCompactMin):
+  // squeeze them out.  Compaction is an optimisation — if it cannot get the
+  // device memory it leaves the store as it is.
+  const size_t dead = nphys_ - nalive_;
+  if (dead >= kCompactMin && 4 * dead > nphys_) {
+    // the delete is done whatever happens here
+    int rc;
+    try {
+      rc = compact();
+    } catch (const std::bad_alloc&) {
+      set_error(SVDB_ENOMEM, "host allocation failed");
+      rc = SVDB_ENOMEM;
+    }
+    if (rc != SVDB_OK) fprintf(stderr, "svdb: compaction skipped: %s\n", last_error());
+  }
+  return SVDB_OK;
+}
+
+// Moves the live rows to slots 0..nalive-1 in order (slot order is logical
+// order, so this is a stable gather), recomputes their norms, and rebuilds
+// the shadows from the compacted master: requantized rows, and the
+// degenerate list and ρ from the live rows only (both otherwise only grow).
+int Engine::compact() {
+  if (nalive_ == nphys_) return SVDB_OK;
+  fault_point();
+  std::vector<uint32_t> keep;
+  keep.reserve(nalive_);
+  for (size_t s = 0; s < nphys_; s++)
+    if ((h_alive_[s >> 6] >> (s & 63)) & 1ull) keep.push_back((uint32_t)s);
+  const size_t xb = cap_ * ld_ * sizeof(double), nb = cap_ * sizeof(double);
+  double *X = nullptr, *nx2 = nullptr, *xn = nullptr;
+  uint32_t* didx = nullptr;
+  const size_t ib = std::max<size_t>(keep.size(), 1) * sizeof(uint32_t);
+  if (dev_alloc((void**)&X, xb) || dev_alloc((void**)&nx2, nb) || dev_alloc((void**)&xn, nb) ||
+      dev_alloc((void**)&didx, ib)) {
+    dev_free(X, xb);
+    dev_free(nx2, nb);
+    dev_free(xn, nb);
+    (void)hipGetLastError();
+    return SVDB_OK;  // not enough device memory: keep the tombstones
+  }
+  const uint32_t n = (uint32_t)keep.size();
+  int rc = SVDB_OK;
+  auto fail = [&](const char* what) {
+    set_error(SVDB_EDEVICE, std::string("compaction: ") + what);
+    rc = SVDB_EDEVICE;
+  };
+  if (hipMemcpyAsync(didx, keep.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream_) !=
+          hipSuccess ||
+      launch_gather_rows(X_, ld_, didx, n, (uint32_t)ld_, X, ld_, stream_) != hipSuccess ||
+      launch_row_norms(X, ld_, nullptr, 0, n, (uint32_t)d_, nx2, xn, stream_) != hipSuccess ||
+      hipStreamSynchronize(stream_) != hipSuccess)
+    fail("row gather");
+  dev_free(didx, ib);
+  if (rc) {
+    dev_free(X, xb);
+    dev_free(nx2, nb);
+    dev_free(xn, nb);
+    return rc;
+  }
+  dev_free(X_, xb);
+  dev_free(nx2_, nb);
+  dev_free(xn_, nb);
+  X_ = X;
+  nx2_ = nx2;
+  xn_ = xn;
+  // host bitmap: slots 0..n-1 alive
+  const size_t ow = (nphys_ + 63) / 64;
+  for (size_t w = 0; w < ow; w++) {
+    const size_t lo = w * 64;
+    h_alive_[w] = lo + 64 <= n ? ~0ull : lo >= n ? 0ull : ((1ull << (n - lo)) - 1ull);
+    h_rank_[w] = (uint32_t)std::min<size_t>(lo, n);
+  }
+  nphys_ = n;
+  has_deletes_ = false;
+  dirty_word_ = 0;
+  // shadows: every slot dead and zeroed, ρ and the degenerate list reset,
+  // then the live rows requantized.  Should a step fail, the shadows are
+  // switched off (a half-rebuilt shadow could hide rows from the first
+  // pass) and searches use the FP64 scan.
+  bool ok = true;
+  if (lowp_)
+    ok = ok && hipMemsetAsync(sh_.Xb, 0, cap_ * sh_.dp * sizeof(uint16_t), stream_) == hipSuccess &&
+         hipMemsetAsync(sh_.rho_bits, 0, 3 * sizeof(uint32_t), stream_) == hipSuccess &&
+         launch_shadow_init(sh_, 0, (uint32_t)cap_, stream_) == hipSuccess;
+  if (lowp8_)
+    ok = ok && hipMemsetAsync(sh8_.X8, 0, cap_ * sh8_.dp, stream_) == hipSuccess &&
+         hipMemsetAsync(sh8_.rho_bits, 0, 3 * sizeof(uint32_t), stream_) == hipSuccess &&
+         launch_shadow_init8(sh8_, 0, (uint32_t)cap_, stream_) == hipSuccess;
+  ok = ok && shadow_rows(0, n, nullptr) == SVDB_OK &&
+       hipStreamSynchronize(stream_) == hipSuccess;
+  if (!ok) {
+    lowp_ = lowp8_ = false;
+    lowp_wanted_ = false;
+    fprintf(stderr, "svdb: shadow rebuild after compaction failed; FP64 scan only\n");
+  }
+  degen_dirty_ = false;
+  st_.compactions++;
   return SVDB_OK;
 }
 
@@ -846,6 +985,7 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   uint32_t* h_counts = static_cast<uint32_t*>(h_io_);
   double* h_err = reinterpret_cast<double*>(static_cast<char*>(h_io_) + round_up(nr * sizeof(uint32_t), 64));
   if ((rc = sync_alive())) return rc;
+  if ((rc = refresh_degen())) return rc;
 
   HIPCHK(hipMemsetAsync(s_err_, 0, (size_t)nq * sizeof(double), stream_), "memset");
   // query prep per range; each range's slots run to the next range (the last
@@ -1199,6 +1339,7 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   int rc;
   for (size_t i = 0; i < nq; i++)
     if ((rc = check_search(d_, D, metric[i], precision))) return rc;
+  fault_point();
   const int which = nphys_ > 0 ? use_lowp(precision, D, &rc) : 0;
   if (rc) return rc;
   const bool lowp = which != 0;
@@ -1334,6 +1475,7 @@ int Engine::pairs_f64(const size_t* a, const size_t* b, size_t np, int metric, d
     return SVDB_EINVAL;
   }
   if (np == 0) return SVDB_OK;
+  fault_point();
   std::vector<uint32_t> hs(2 * np);
   for (size_t p = 0; p < np; p++) {
     if (a[p] >= nalive_ || b[p] >= nalive_) {

@@ -10,7 +10,8 @@
 // Slot order therefore equals logical (positional) order, so the lowest
 // slot on a tie is the lowest logical index, and a logical index is the
 // rank of its slot among the alive slots.  Compaction squeezes tombstones
-// out when they pass a quarter of the slots.
+// out when they pass a quarter of the slots (and number at least 256), and
+// rebuilds the shadows, their degenerate lists and ρ from the live rows.
 //
 // Every public method takes the engine mutex: callers may enter from any
 // thread (the reference server runs one thread per connection).
@@ -32,6 +33,13 @@ namespace svdb {
 
 void set_error(int code, const std::string& msg);
 const char* last_error();
+
+// Host-allocation fault injection for tests (svdb_debug_fail_host_alloc):
+// placed before host allocations that may fail; when armed, the n-th call
+// throws std::bad_alloc, as a failed allocation there would.
+void fault_point();
+void arm_fault(int64_t n);  // 0 disarms
+int64_t fault_armed();     // calls left before the fault (0: disarmed)
 
 // The reference's float arithmetic for one pair of host vectors of length n
 // (vector_database.c:301-352), computed on the GPU: both vectors are staged
@@ -56,6 +64,7 @@ struct KernelTime {
 
 struct EngineStats {
   uint64_t searches = 0, queries = 0, certified = 0, fallbacks = 0, widened = 0, second_pass = 0;
+  uint64_t compactions = 0;
   double last_max_rel_err = 0.0;
 };
 
@@ -106,6 +115,9 @@ class Engine {
   int ensure_dim(size_t d);
   int ensure_capacity(size_t slots);
   int sync_alive();
+  int refresh_degen();
+  int compact();
+  static constexpr size_t kCompactMin = 256;  // fewest tombstones worth a compaction
   int dev_alloc(void** p, size_t bytes);
   void dev_free(void* p, size_t bytes);
   int scratch(void** p, size_t* have, size_t need);
@@ -152,6 +164,7 @@ class Engine {
   std::vector<uint64_t> h_alive_;
   std::vector<uint32_t> h_rank_;
   bool has_deletes_ = false;
+  bool degen_dirty_ = false;  // degenerate lists may hold deleted / rewritten slots
   size_t dirty_word_ = SIZE_MAX;  // first bitmap word not yet on the device
 
   // low-precision shadow

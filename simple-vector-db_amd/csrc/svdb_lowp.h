@@ -190,6 +190,10 @@ hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s);
 constexpr int kMaxLaunchRanges = 4;
 hipError_t launch_shortlist_ranges(const ShortlistArgs* a, int n, hipStream_t s);
 hipError_t launch_rerank_ranges(const RerankArgs* a, const int* metric, int n, hipStream_t s);
+// Rebuilds a shadow's degenerate list (count, list[kDegenCap]) from the
+// live slots whose penalty is +inf.
+hipError_t launch_rebuild_degen(const float* pen, const uint64_t* alive, uint32_t nslots,
+                                uint32_t* count, uint32_t* list, hipStream_t s);
 hipError_t launch_gather_rows(const double* src, size_t lds, const uint32_t* idx, uint32_t n,
                               uint32_t d, double* dst, size_t ldd, hipStream_t s);
 hipError_t launch_scatter_results(const uint64_t* ids, const double* sc, uint32_t k,

@@ -870,6 +870,36 @@ hipError_t launch_rerank(const RerankArgs& a, int metric, hipStream_t s) {
 
 // ------------------------------------------------------------- fallback plumbing
 
+// Rebuilds a shadow's degenerate-row list from the store: a live slot whose
+// penalty is +inf holds a degenerate row (dead slots carry +inf too, and
+// are skipped).  Slots deleted or rewritten since their row entered the
+// list drop out, so the list holds only rows that are still degenerate.
+// `count` must be zero on entry (k_zero_u32); the list's order is
+// immaterial (the rerank ranks its rows by key and slot).
+__global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void k_collect_degen(const float* __restrict__ pen, const uint64_t* __restrict__ alive,
+                                uint32_t nslots, uint32_t* __restrict__ count,
+                                uint32_t* __restrict__ list) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  if (!((alive[s >> 6] >> (s & 63)) & 1ull) || !(pen[s] == INFINITY)) return;
+  const uint32_t idx = atomicAdd(count, 1u);
+  if (idx < (uint32_t)kDegenCap) list[idx] = s;
+}
+
+hipError_t launch_rebuild_degen(const float* pen, const uint64_t* alive, uint32_t nslots,
+                                uint32_t* count, uint32_t* list, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, count, 1u);
+  if (nslots)
+    hipLaunchKernelGGL(k_collect_degen, dim3((nslots + 255) / 256), dim3(256), 0, s, pen, alive,
+                       nslots, count, list);
+  return hipGetLastError();
+}
+
 __global__ void k_gather_rows(const double* __restrict__ src, size_t lds,
                               const uint32_t* __restrict__ idx, uint32_t n, uint32_t d,
                               double* __restrict__ dst, size_t ldd) {

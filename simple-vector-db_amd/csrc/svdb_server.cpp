@@ -13,11 +13,16 @@
 // device thread takes it.  A mutation is a barrier: searches queued
 // before it see the store without it, searches after it see it.  The host
 // mirror (db->vectors) follows the device store on a third thread, in the
-// same order; a mutation completes once both halves are applied.
+// same order; a mutation completes once both halves are applied, and a
+// search completes only once the mirror holds every mutation submitted
+// before it, so the ids it returns name the same rows in vector_db_read.
+// No exception leaves a worker thread: a failed host allocation fails the
+// request it was made for (SVDB_ENOMEM).
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <memory>
 #include <new>
 #include <thread>
@@ -60,6 +65,9 @@ struct Tick {
   // A tick is either a run of consecutive mutations (applied in order) or
   // a batch of searches.
   std::vector<std::shared_ptr<Request>> mutations;
+  // searches: mutations handed to the mirror before this tick ran; the
+  // tick completes once the mirror has applied that many
+  uint64_t mirror_seq = 0;
 };
 
 class Server {
@@ -162,10 +170,23 @@ class Server {
       cv_q_.wait(g, [&] { return stop_ || can_stage(); });
       if (stop_) return;
       if (!staged_ && queue_.front()->kind != 0) {
-        std::unique_ptr<Tick> t(new Tick);
+        std::unique_ptr<Tick> t(new (std::nothrow) Tick);
+        bool room = t != nullptr;
+        if (room) {
+          try {
+            fault_point();
+            t->mutations.reserve(kMaxMutationsPerTick);
+          } catch (const std::bad_alloc&) {
+            room = false;
+          }
+        }
+        if (!room) {  // no host memory for the tick: fail the head request
+          fail_head(SVDB_ENOMEM);
+          continue;
+        }
         while (!queue_.empty() && queue_.front()->kind != 0 &&
                t->mutations.size() < kMaxMutationsPerTick) {
-          t->mutations.push_back(queue_.front());
+          t->mutations.push_back(queue_.front());  // capacity reserved
           queue_.pop_front();
         }
         staged_ = std::move(t);
@@ -173,8 +194,12 @@ class Server {
         std::unique_ptr<Tick> fresh;
         Tick* t = staged_.get();
         if (!t) {
-          fresh.reset(new Tick);
+          fresh.reset(new (std::nothrow) Tick);
           t = fresh.get();
+          if (!t) {
+            fail_head(SVDB_ENOMEM);
+            continue;
+          }
         }
         const size_t d = eng_->dim();
         while (!queue_.empty() && queue_.front()->kind == 0) {
@@ -192,6 +217,7 @@ class Server {
           queue_.pop_front();
           const size_t o = t->nq;
           try {  // room first: a host allocation failure fails this request only
+            fault_point();
             t->q.resize((o + r->nq) * d);
             t->k.resize(o + r->nq, r->k);
             t->metric.resize(o + r->nq, r->metric);
@@ -214,6 +240,16 @@ class Server {
     }
   }
 
+  // (under mu_) fails the request at the head of the queue
+  void fail_head(int rc) {
+    auto r = queue_.front();
+    queue_.pop_front();
+    r->rc = rc;
+    r->t_start = r->t_done = now_us();
+    r->done = true;
+    cv_done_.notify_all();
+  }
+
   // (under mu_) whether the stager has work: a queue head that starts a new
   // tick, or a search that joins the staged search tick
   bool can_stage() const {
@@ -234,6 +270,7 @@ class Server {
         cv_tick_.wait(g, [&] { return stop_ || staged_; });
         if (!staged_ && stop_) return;
         t = std::move(staged_);
+        t->mirror_seq = mirror_issued_;
       }
       cv_q_.notify_all();  // the stager may form the next tick now
       const int64_t t0 = now_us();
@@ -241,16 +278,29 @@ class Server {
       if (!t->mutations.empty()) {
         // Device halves now (the next tick's searches must see them); host
         // mirror halves, in the same order, on the mirror thread, which
-        // completes the requests.
+        // completes the requests.  The mirror queue's node is allocated
+        // before the device half runs, so a mutation applied on the device
+        // always reaches the mirror.
         for (auto& m : t->mutations) {
-          run_mutation(*m);
-          if (m->rc != SVDB_OK) {
+          std::list<std::shared_ptr<Request>> node;
+          try {
+            fault_point();
+            node.push_back(m);
+            run_mutation(*m);
+          } catch (const std::bad_alloc&) {
+            m->rc = SVDB_ENOMEM;
+          } catch (...) {
+            m->rc = SVDB_EINVAL;
+          }
+          if (m->rc != SVDB_OK || node.empty()) {
+            if (m->rc == SVDB_OK) m->rc = SVDB_ENOMEM;
             finish(m, t0, tick, 0);
           } else {
             std::lock_guard<std::mutex> g(mu_);
             m->t_start = t0;
             m->tick = tick;
-            mirror_q_.push_back(m);
+            mirror_q_.splice(mirror_q_.end(), node);  // allocates nothing
+            mirror_issued_++;
           }
         }
         cv_mirror_.notify_all();
@@ -261,20 +311,23 @@ class Server {
       std::vector<double> sc;
       int rc = SVDB_OK;
       try {  // a host allocation failure fails the tick's searches
+        fault_point();
         ids.resize(t->nq * t->kmax);
         sc.resize(t->nq * t->kmax);
         if (t->nq)
           rc = eng_->search_host(t->q.data(), t->nq, d, d, t->k.data(), t->metric.data(),
                                  cfg_.precision, t->kmax, ids.data(), sc.data());
       } catch (const std::bad_alloc&) {
-        set_error(SVDB_ENOMEM, "host allocation failed");
         rc = SVDB_ENOMEM;
+      } catch (...) {
+        rc = SVDB_EINVAL;
       }
       size_t o = 0;
       for (auto& r : t->reqs) {
         r->rc = rc;
         if (rc == SVDB_OK) {
           try {
+            fault_point();
             r->ids.resize(r->nq * r->k);
             r->scores.resize(r->nq * r->k);
           } catch (const std::bad_alloc&) {
@@ -289,6 +342,12 @@ class Server {
             }
         }
         o += r->nq;
+      }
+      // the ids name rows of the store as the mutations before this tick
+      // left it: the mirror (vector_db_read) must show that state first
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_applied_.wait(g, [&] { return mirror_applied_ >= t->mirror_seq; });
       }
       finish(t->reqs, t0, tick, t->nq);
     }
@@ -313,7 +372,11 @@ class Server {
     }
   }
 
-  // Host-mirror half, in mutation order; completes the request.
+  // Host-mirror half, in mutation order; completes the request.  The
+  // mirror functions allocate nothing that can throw (malloc / realloc,
+  // reserved ahead by mirror_reserve); a row whose copy cannot be
+  // allocated is mirrored with a NULL data pointer, which the reference's
+  // handlers treat as an invalid row.
   void mirror_loop() {
     for (;;) {
       std::shared_ptr<Request> r;
@@ -339,6 +402,11 @@ class Server {
       } else {
         mirror_delete(db_, r->index);
       }
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        mirror_applied_++;
+      }
+      cv_applied_.notify_all();
       finish(r, r->t_start, r->tick, 0);
     }
   }
@@ -372,9 +440,11 @@ class Server {
   Engine* eng_;
   svdb_server_config cfg_;
   std::mutex mu_;
-  std::condition_variable cv_q_, cv_tick_, cv_done_, cv_mirror_;
+  std::condition_variable cv_q_, cv_tick_, cv_done_, cv_mirror_, cv_applied_;
   std::deque<std::shared_ptr<Request>> queue_;
-  std::deque<std::shared_ptr<Request>> mirror_q_;
+  std::list<std::shared_ptr<Request>> mirror_q_;
+  uint64_t mirror_issued_ = 0;   // mutations handed to the mirror thread
+  uint64_t mirror_applied_ = 0;  // ... and applied by it
   bool mirror_stop_ = false;
   std::unique_ptr<Tick> staged_;
   std::unordered_map<int64_t, std::shared_ptr<Request>> all_;

@@ -34,7 +34,7 @@ EXPORTS = [
     "svdb_device_free", "svdb_device_copy", "svdb_device_sync", "svdb_server_default_config",
     "svdb_server_start", "svdb_server_stop", "svdb_server_submit_search",
     "svdb_server_submit_insert", "svdb_server_submit_update", "svdb_server_submit_delete",
-    "svdb_server_wait",
+    "svdb_server_wait", "svdb_debug_fail_host_alloc",
 ]
 
 
@@ -58,7 +58,8 @@ class Stats(C.Structure):
                 ("device_bytes", C.c_size_t), ("budget_bytes", C.c_size_t),
                 ("searches", C.c_uint64), ("queries", C.c_uint64), ("certified", C.c_uint64),
                 ("fallbacks", C.c_uint64), ("last_max_rel_err", C.c_double),
-                ("widened", C.c_uint64), ("second_pass", C.c_uint64)]
+                ("widened", C.c_uint64), ("second_pass", C.c_uint64),
+                ("compactions", C.c_uint64)]
 
 
 class KernelTime(C.Structure):
@@ -150,12 +151,19 @@ def lib():
     L.svdb_server_submit_delete.argtypes = [vp, sz]
     L.svdb_server_wait.restype = C.c_int
     L.svdb_server_wait.argtypes = [vp, C.c_int64, vp, vp, C.POINTER(RequestInfo)]
+    L.svdb_debug_fail_host_alloc.restype = C.c_int64
+    L.svdb_debug_fail_host_alloc.argtypes = [C.c_int64]
     _libc = C.CDLL(None)
     _libc.malloc.restype = C.c_void_p
     _libc.malloc.argtypes = [C.c_size_t]
     _libc.free.argtypes = [C.c_void_p]
     _lib = L
     return L
+
+
+def fail_host_alloc(n):
+    """Test hook: the n-th host allocation point from now fails (0 disarms)."""
+    return lib().svdb_debug_fail_host_alloc(n)
 
 
 class SvdbError(RuntimeError):

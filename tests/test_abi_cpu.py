@@ -52,3 +52,9 @@ def test_no_gpu_fails_loudly():
     assert not svdb.lib().vector_db_init(0, 3)
     assert not svdb.lib().kdtree_create(3)
     assert b"HIP device" in svdb.lib().svdb_last_error() or svdb.lib().svdb_last_error()
+
+
+def test_init_refuses_an_overflowing_capacity():
+    # capacity * sizeof(Vector) would wrap: refused before any allocation
+    # (vector_database.c:17-54 has the same multiplication unchecked)
+    assert not svdb.lib().vector_db_init((1 << 64) // 32, 3)
