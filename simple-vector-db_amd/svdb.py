@@ -34,7 +34,7 @@ EXPORTS = [
     "svdb_device_free", "svdb_device_copy", "svdb_device_sync", "svdb_server_default_config",
     "svdb_server_start", "svdb_server_stop", "svdb_server_submit_search",
     "svdb_server_submit_insert", "svdb_server_submit_update", "svdb_server_submit_delete",
-    "svdb_server_wait", "svdb_debug_fail_host_alloc",
+    "svdb_server_wait", "svdb_debug_fail_host_alloc", "svdb_nearest_request",
 ]
 
 
@@ -151,6 +151,9 @@ def lib():
     L.svdb_server_submit_delete.argtypes = [vp, sz]
     L.svdb_server_wait.restype = C.c_int
     L.svdb_server_wait.argtypes = [vp, C.c_int64, vp, vp, C.POINTER(RequestInfo)]
+    L.svdb_nearest_request.restype = C.c_int
+    L.svdb_nearest_request.argtypes = [DBP, vp, sz, C.c_char_p, sz, C.c_char_p, C.c_char_p,
+                                       C.c_int, C.POINTER(C.c_char_p), C.POINTER(sz)]
     L.svdb_debug_fail_host_alloc.restype = C.c_int64
     L.svdb_debug_fail_host_alloc.argtypes = [C.c_int64]
     _libc = C.CDLL(None)
@@ -159,6 +162,29 @@ def lib():
     _libc.free.argtypes = [C.c_void_p]
     _lib = L
     return L
+
+
+def free_response(p):
+    """free() a response text returned by svdb_nearest_request."""
+    lib()
+    _libc.free(C.cast(p, C.c_void_p))
+
+
+def nearest_request(db, body, expected_dim, number=None, metric=None, precision="auto",
+                    server=None):
+    """svdb_nearest_request -> (HTTP status, response text)."""
+    out = C.c_char_p()
+    n = C.c_size_t(0)
+    raw = body.encode() if isinstance(body, str) else body
+    st = lib().svdb_nearest_request(db.p if db is not None else None,
+                                    server.p if server is not None else None, expected_dim,
+                                    raw, len(raw),
+                                    number.encode() if number is not None else None,
+                                    metric.encode() if metric is not None else None,
+                                    _prec(precision), C.byref(out), C.byref(n))
+    text = C.string_at(out, n.value).decode()
+    free_response(out)
+    return st, text
 
 
 def fail_host_alloc(n):
