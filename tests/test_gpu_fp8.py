@@ -201,3 +201,33 @@ def test_bf16_wide_512_mutations():
         assert_same(ids, sc, rid, rsc, f"bf16 512 {metric}")
     assert db.stats()["certified"] > 0
     db.close()
+
+
+@pytest.mark.parametrize("n,d", [(200_000, 768), (100_000, 1536)])
+def test_fp8_wide_workload_shape(n, d):
+    """The wide-vector workload's shape at a size the oracle can check: a
+    cluster mixture (1,024 centres, σ 0.05, LogNormal(0, 1) norms), one batch
+    of 4,096 queries, top-100, a 128-wide shortlist (the widening retries
+    and the BF16 second pass behind it).  All 4,096 answers equal the FP64
+    GPU scan's; 256 of them are also checked against the CPU FP64 oracle."""
+    X = workloads.mixture(n, d, seed=61, n_centers=1024, lognorm=1.0)
+    Q = workloads.mixture(4096, d, seed=62, n_centers=1024, lognorm=1.0)
+    db = store(X)
+    db.set_shortlist(128)
+    sample = np.random.default_rng(63).choice(4096, 256, replace=False)
+    for metric in ("dot", "l2", "cos"):
+        say(f"fp8 {n}x{d} {metric} k=100 L=128, 4096 queries")
+        s0 = db.stats()
+        ids, sc = db.search(Q, 100, metric, precision="fp8")
+        s1 = db.stats()
+        fids, fsc = db.search(Q, 100, metric, precision="fp64")
+        assert_same(ids, sc, fids, fsc, f"fp8 vs fp64 scan {d} {metric}")
+        rid, rsc = O.topk(X, Q[sample], 100, metric)
+        assert_same(ids[sample], sc[sample], rid, rsc, f"fp8 vs oracle {d} {metric}")
+        cert = s1["certified"] - s0["certified"]
+        wid = s1["widened"] - s0["widened"]
+        sec = s1["second_pass"] - s0["second_pass"]
+        fb = s1["fallbacks"] - s0["fallbacks"]
+        say(f"  certified {cert} (stages counted as they certify: {wid} only after a wider "
+            f"retry; {sec} handed to the BF16 pass, counted there again), FP64 rescans {fb}")
+    db.close()

@@ -87,8 +87,11 @@ def test_mixed_ticks_equal_alone():
     final_mixed = len(db)
     db.close()
 
-    # alone: the same requests one at a time through svdb_search
+    # alone: the same requests one at a time through svdb_search; and every
+    # served search against the FP64 oracle on the rows as the mutations
+    # before it left them (a list model replayed in submission order)
     db = _store(rows)
+    model = rows.copy()
     batched = 0
     for p, (rc, ids, sc, info) in zip(plan, mixed):
         assert rc == 0, p[0]
@@ -97,14 +100,20 @@ def test_mixed_ticks_equal_alone():
             ids_a, sc_a = db.search(Q, k, m, precision="auto")
             assert np.array_equal(ids, ids_a), (Q.shape, k, m)
             assert np.array_equal(_bits(sc), _bits(sc_a)), (Q.shape, k, m)
+            ids_o, sc_o = O.topk(model, Q, k, m)
+            assert np.array_equal(ids, ids_o), ("oracle", Q.shape, k, m)
+            assert np.array_equal(_bits(sc), _bits(sc_o)), ("oracle", Q.shape, k, m)
             batched += info["tick_queries"] > Q.shape[0]
         elif p[0] == "insert":
-            assert db.insert(p[1]) == ids
+            assert db.insert(p[1]) == ids == model.shape[0]
+            model = np.vstack([model, p[1]])
         elif p[0] == "update":
             db.update(p[1], p[2])
+            model[p[1]] = p[2]
         else:
             db.delete(p[1])
-    assert len(db) == final_mixed
+            model = np.delete(model, p[1], axis=0)
+    assert len(db) == final_mixed == model.shape[0]
     assert batched > 0  # some requests really shared a tick
     # the final store agrees with the FP64 oracle
     Q = pool[:64]
@@ -221,3 +230,68 @@ def test_concurrent_submitters():
             db.update(p[1], p[2])
     assert shared > 0
     db.close()
+
+
+def test_serving_mix_on_the_fp64_scan_store():
+    """The serving workload over the FP64-scan workload's store shape:
+    1,000,000 x 128 rows i.i.d. N(0,1) (seed 1), Poisson-free burst of 60
+    requests with log-uniform 1..512 queries, k in {1, 10, 100}, metrics
+    uniform, 5 % mutations; the BF16 path (the server's default).  Each
+    request's first queries (up to 4) are checked against the FP64 oracle
+    on the rows as they stood when it was admitted, ids and score bits."""
+    rng = np.random.default_rng(1)
+    rows = rng.standard_normal((1_000_000, D))
+    qrng = np.random.default_rng(2)
+    reqs = _requests(np.random.default_rng(12), 60)
+    plan, size = [], rows.shape[0]
+    for r in reqs:
+        if r[0] == "search":
+            _, nq, k, m = r
+            plan.append(("search", qrng.standard_normal((nq, D)), k, m))
+        elif r[1] == "insert":
+            plan.append(("insert", qrng.standard_normal(D)))
+            size += 1
+        elif r[1] == "update":
+            plan.append(("update", int(qrng.integers(size)), qrng.standard_normal(D)))
+        else:
+            plan.append(("delete", int(qrng.integers(size))))
+            size -= 1
+    print("serving mix on 1M x 128 N(0,1):", sum(p[0] == "search" for p in plan), "searches,",
+          sum(p[0] != "search" for p in plan), "mutations", flush=True)
+    db = _store(rows)
+    srv = svdb.Server(db)
+    tickets = []
+    for p in plan:
+        if p[0] == "search":
+            tickets.append(srv.submit_search(p[1], p[2], p[3]))
+        elif p[0] == "insert":
+            tickets.append(srv.submit_insert(p[1]))
+        elif p[0] == "update":
+            tickets.append(srv.submit_update(p[1], p[2]))
+        else:
+            tickets.append(srv.submit_delete(p[1]))
+    out = [srv.wait(t) for t in tickets]
+    srv.stop()
+    stats = db.stats()
+    db.close()
+    model = rows
+    checked = 0
+    for p, (rc, ids, sc, info) in zip(plan, out):
+        assert rc == 0, p[0]
+        if p[0] == "search":
+            Q, k, m = p[1], p[2], p[3]
+            n = min(4, Q.shape[0])
+            ids_o, sc_o = O.topk(model, Q[:n], k, m)
+            assert np.array_equal(ids[:n], ids_o), (Q.shape, k, m)
+            assert np.array_equal(_bits(sc[:n]), _bits(sc_o)), (Q.shape, k, m)
+            checked += n
+        elif p[0] == "insert":
+            assert ids == model.shape[0]
+            model = np.vstack([model, p[1]])
+        elif p[0] == "update":
+            model = model.copy() if model is rows else model
+            model[p[1]] = p[2]
+        else:
+            model = np.delete(model, p[1], axis=0)
+    assert checked >= 100
+    assert stats["certified"] > 0
