@@ -54,6 +54,18 @@ def lib():
         L.orc_pair_fp64.restype = C.c_double
         L.orc_pair_fp64.argtypes = [dp, dp, sz, C.c_int]
         L.orc_pairs_fp64.argtypes = [dp, sz, sz, u64p, u64p, sz, C.c_int, dp]
+        L.orc_reftree_build.restype = vp
+        L.orc_reftree_build.argtypes = [dp, sz, sz, sz, C.c_int]
+        L.orc_reftree_handle.restype = vp
+        L.orc_reftree_handle.argtypes = [vp]
+        L.orc_reftree_at.restype = sz
+        L.orc_reftree_at.argtypes = [vp, C.c_char_p, sz]
+        L.orc_reftree_shape.argtypes = [vp, C.POINTER(sz), C.POINTER(sz)]
+        L.orc_reftree_free.argtypes = [vp]
+        L.orc_time_kd.restype = C.c_double
+        L.orc_time_kd.argtypes = [vp, vp, dp, sz, sz, C.c_int, u64p, dp]
+        L.orc_time_pairs.restype = C.c_double
+        L.orc_time_pairs.argtypes = [vp, dp, sz, sz, u64p, u64p, sz, C.c_int, vp]
         _lib = L
     return _lib
 
@@ -169,3 +181,70 @@ class KDTree:
         if getattr(self, "h", None):
             lib().orc_kd_free(self.h)
             self.h = None
+
+
+# ---- the reference's own compiled CPU path (oracle/_ref), timed ----------
+
+REF_BUILDS = {"as_shipped": REFLIB, "O2": os.path.join(HERE, "_ref", "libsvdbref_O2.so")}
+
+
+def ref_lib(build="as_shipped"):
+    """The reference's kdtree.c + vector_database.c built from its sources
+    (oracle/Makefile `ref`), or None where that build is absent."""
+    path = REF_BUILDS[build]
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.kdtree_nearest.restype = C.c_size_t
+    L.kdtree_nearest.argtypes = [C.c_void_p, C.c_void_p]
+    return L
+
+
+def fn_addr(L, name):
+    return C.cast(getattr(L, name), C.c_void_p).value
+
+
+class RefTree:
+    """The tree the reference's kdtree_insert grows from rows 0..n-1, laid out
+    in its own KDTreeNode / KDTree structures (refkd_bench.c)."""
+
+    def __init__(self, X, dimension, threads=1):
+        X = _f64(X)
+        self.h = lib().orc_reftree_build(X.ctypes.data, X.shape[0], X.shape[1], dimension,
+                                         int(threads))
+        if not self.h:
+            raise MemoryError("orc_reftree_build failed")
+        self.tree = lib().orc_reftree_handle(self.h)
+
+    def at(self, path):
+        return lib().orc_reftree_at(self.tree, bytes(path), len(path))
+
+    def shape(self):
+        c, h = C.c_size_t(0), C.c_size_t(0)
+        lib().orc_reftree_shape(self.tree, C.byref(c), C.byref(h))
+        return c.value, h.value
+
+    def time_nearest(self, fn, Q, threads):
+        """fn: address of a kdtree_nearest.  -> (ids, per-query ns, wall s)."""
+        Q = _f64(np.atleast_2d(Q))
+        ids = np.empty(Q.shape[0], dtype=np.uint64)
+        ns = np.empty(Q.shape[0], dtype=np.float64)
+        wall = lib().orc_time_kd(fn, self.tree, Q.ctypes.data, Q.shape[0], Q.shape[1],
+                                 int(threads), ids.ctypes.data, ns.ctypes.data)
+        return ids, ns, wall
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_reftree_free(self.h)
+            self.h = None
+
+
+def time_ref_pairs(fn, A, ia, ib, threads):
+    """fn: address of a reference metric (cosine_similarity, ...).  -> (float out, wall s)."""
+    A = _f64(A)
+    ia = np.ascontiguousarray(ia, dtype=np.uint64)
+    ib = np.ascontiguousarray(ib, dtype=np.uint64)
+    out = np.empty(ia.shape[0], dtype=np.float32)
+    wall = lib().orc_time_pairs(fn, A.ctypes.data, A.shape[1], A.shape[1], ia.ctypes.data,
+                                ib.ctypes.data, ia.shape[0], int(threads), out.ctypes.data)
+    return out, wall

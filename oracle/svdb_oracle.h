@@ -81,6 +81,24 @@ double orc_pair_fp64(const double* a, const double* b, size_t D, int metric);
 void   orc_pairs_fp64(const double* A, size_t lda, size_t D, const uint64_t* ia,
                       const uint64_t* ib, size_t np, int metric, double* out);
 
+/* ---- the reference's own CPU path, timed (refkd_bench.c) -------------- *
+ * orc_reftree_build lays out, in the reference's KDTreeNode / KDTree
+ * structures, exactly the tree kdtree_insert grows from rows 0..n-1 in
+ * order (kdtree.c:47-62), with nthreads threads; orc_reftree_handle gives
+ * the KDTree* to pass to the reference's kdtree_nearest.  orc_time_kd /
+ * orc_time_pairs call a reference function through its address on
+ * nthreads independent streams and return wall seconds. */
+typedef struct orc_reftree orc_reftree;
+orc_reftree* orc_reftree_build(const double* X, size_t n, size_t ldx, size_t D, int nthreads);
+void* orc_reftree_handle(orc_reftree* t);
+size_t orc_reftree_at(const void* tree, const unsigned char* path, size_t len);
+void orc_reftree_shape(const void* tree, size_t* count, size_t* height);
+void orc_reftree_free(orc_reftree* t);
+double orc_time_kd(void* fn, void* tree, const double* Q, size_t nq, size_t ldq, int nthreads,
+                   uint64_t* ids, double* ns);
+double orc_time_pairs(void* fn, const double* X, size_t ldx, size_t d, const uint64_t* ia,
+                      const uint64_t* ib, size_t np, int nthreads, float* out);
+
 /* ---- helpers used by tests -------------------------------------------- */
 /* Map a score to its ascending rank key exactly as the ranking uses it. */
 double orc_rank_key(double score_or_l2key, int metric);

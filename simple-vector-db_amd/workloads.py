@@ -21,18 +21,41 @@ def centers(n_centers, d, seed):
     return c / np.linalg.norm(c, axis=1, keepdims=True)
 
 
+def _mixture_rows(out, C, rng, sigma, lognorm):
+    m, d = out.shape
+    j = rng.integers(0, C.shape[0], m)
+    u = C[j] + sigma * rng.standard_normal((m, d))
+    r = rng.lognormal(0.0, lognorm, (m, 1))
+    out[:] = r * u / np.linalg.norm(u, axis=1, keepdims=True)
+
+
 def mixture(n, d, seed, n_centers=1024, sigma=0.05, lognorm=0.5, center_seed=3,
-            chunk=1_000_000):
-    """Rows of the cluster mixture, generated in chunks (bounded temporaries)."""
+            chunk=1_000_000, threads=None):
+    """Rows of the cluster mixture.  Up to `chunk` rows: one PCG64 stream from
+    `seed`.  Larger stores: blocks of `chunk` rows, block b drawn from the
+    stream default_rng([seed, b]) (the same rows whatever the thread count),
+    on `threads` threads (default: this process's cores, at most 16)."""
     C = centers(n_centers, d, center_seed)
-    rng = np.random.default_rng(seed)
     out = np.empty((n, d))
-    for s in range(0, n, chunk):
-        m = min(chunk, n - s)
-        j = rng.integers(0, n_centers, m)
-        u = C[j] + sigma * rng.standard_normal((m, d))
-        r = rng.lognormal(0.0, lognorm, (m, 1))
-        out[s:s + m] = r * u / np.linalg.norm(u, axis=1, keepdims=True)
+    if n <= chunk:
+        _mixture_rows(out, C, np.random.default_rng(seed), sigma, lognorm)
+        return out
+    import concurrent.futures as cf
+    import os
+    nb = (n + chunk - 1) // chunk
+    th = threads or min(16, len(os.sched_getaffinity(0)))
+
+    def block(b):
+        s = b * chunk
+        # smaller pieces inside a block keep the temporaries bounded
+        rng = np.random.default_rng([seed, b])
+        e = min(n, s + chunk)
+        step = max(1, (1 << 27) // (8 * d))
+        for p in range(s, e, step):
+            _mixture_rows(out[p:min(e, p + step)], C, rng, sigma, lognorm)
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, th)) as ex:
+        list(ex.map(block, range(nb)))
     return out
 
 

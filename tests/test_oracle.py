@@ -7,6 +7,7 @@ correctly rounded step at a time), numpy brute force, and closed forms
 (duplicates, D=1 order statistics, permutation invariance).
 """
 import hashlib
+import os
 import math
 from fractions import Fraction
 
@@ -310,3 +311,87 @@ def test_pair_fp64_small_d_is_pairwise_tree_of_products():
     for off in (32, 16, 8, 4, 2, 1):
         p = [p[l] + p[l ^ off] for l in range(64)]
     assert O.pair_fp64(a, b, O.DOT) == p[0]
+
+
+# ---- the harness that times the reference's own CPU path ----------------
+
+def _ref_tree_by_inserts(L, X, D):
+    """Grow the tree with the reference's own kdtree_insert (stdout silenced:
+    it prints per node and per level)."""
+    import ctypes as C
+    import sys
+    L.kdtree_create.restype = C.c_void_p
+    L.kdtree_create.argtypes = [C.c_size_t]
+    L.kdtree_insert.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    libc = C.CDLL(None)
+    sys.stdout.flush()
+    libc.fflush(None)
+    saved, null = os.dup(1), os.open(os.devnull, os.O_WRONLY)
+    os.dup2(null, 1)
+    try:
+        t = L.kdtree_create(D)
+        rows = [np.ascontiguousarray(r) for r in X]
+        for i, r in enumerate(rows):
+            L.kdtree_insert(t, r.ctypes.data, i)
+        libc.fflush(None)
+    finally:
+        os.dup2(saved, 1)
+        os.close(null)
+        os.close(saved)
+    return t, rows
+
+
+@pytest.mark.parametrize("D,threads", [(3, 1), (3, 8), (16, 4)])
+def test_reftree_equals_the_tree_kdtree_insert_grows(D, threads):
+    """orc_reftree_build lays out node for node the tree the reference's own
+    kdtree_insert grows (kdtree.c:47-62), walked path by path."""
+    L = O.ref_lib()
+    if L is None:
+        pytest.skip("reference build (oracle/_ref) absent")
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((3000, 16))
+    X[100:140] = X[7]  # duplicates: equal coordinates go right (kdtree.c:55-58)
+    t_ref, keep = _ref_tree_by_inserts(L, X, D)
+    ours = O.RefTree(X, D, threads=threads)
+    assert ours.shape()[0] == 3000
+    # breadth-first over every path of the reference's tree
+    frontier = [b""]
+    seen = 0
+    while frontier:
+        nxt = []
+        for p in frontier:
+            a = O.lib().orc_reftree_at(t_ref, p, len(p))
+            b = ours.at(p)
+            assert a == b, (p, a, b)
+            if a != (1 << 64) - 1:
+                seen += 1
+                nxt += [p + b"\x00", p + b"\x01"]
+        frontier = nxt
+    assert seen == 3000
+
+
+def test_reference_nearest_through_the_harness(c1_data, golden):
+    """The reference's compiled kdtree_nearest over the harness's tree gives
+    the reference's own C1 answers (golden, from trees grown by inserts), in
+    both builds and on several threads."""
+    X, Q, _ = c1_data
+    for build in ("as_shipped", "O2"):
+        L = O.ref_lib(build)
+        if L is None:
+            pytest.skip("reference build (oracle/_ref) absent")
+        t = O.RefTree(X, 3, threads=4)
+        ids, ns, wall = t.time_nearest(O.fn_addr(L, "kdtree_nearest"), Q, threads=4)
+        assert np.array_equal(ids, golden["c1_kd3_ids"]), build
+        assert wall > 0 and (ns > 0).all()
+
+
+def test_reference_metrics_through_the_harness(c1_data, golden):
+    X, _, P = c1_data
+    for build in ("as_shipped", "O2"):
+        L = O.ref_lib(build)
+        if L is None:
+            pytest.skip("reference build (oracle/_ref) absent")
+        for col, name in enumerate(("cosine_similarity", "euclidean_distance", "dot_product")):
+            out, wall = O.time_ref_pairs(O.fn_addr(L, name), X, P[:, 0], P[:, 1], 3)
+            assert np.array_equal(out.view(np.uint32), golden["c1_metric_bits"][:, col]), \
+                (build, name)
