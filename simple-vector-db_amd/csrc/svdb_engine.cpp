@@ -14,13 +14,22 @@ namespace svdb {
 // ------------------------------------------------------------------ errors
 
 static thread_local std::string g_err;
+// a static message set without allocating (the allocation-free entry point's
+// errors); takes precedence over g_err until the next set_error
+static thread_local const char* g_err_static = nullptr;
 
 void set_error(int code, const std::string& msg) {
   (void)code;
+  g_err_static = nullptr;
   g_err = msg;
 }
 
-const char* last_error() { return g_err.c_str(); }
+void set_error_static(int code, const char* msg) {
+  (void)code;
+  g_err_static = msg;
+}
+
+const char* last_error() { return g_err_static ? g_err_static : g_err.c_str(); }
 
 // Countdown of the armed fault: 0 = disarmed; n > 0 = the n-th fault_point
 // from now throws.  Process-wide.
@@ -126,6 +135,11 @@ Engine::Engine(size_t initial_slots) {
     set_error(SVDB_EDEVICE, "hipStreamCreate failed");
     return;
   }
+  for (auto& c : ctx_)
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+      set_error(SVDB_EDEVICE, "hipStreamCreate failed");
+      return;
+    }
   init_slots_ = round_up(std::max<size_t>(initial_slots, 1024), 64);
   ok_ = true;
 }
@@ -134,22 +148,30 @@ Engine::~Engine() {
   if (!ok_) return;
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X_, nx2_, xn_, alive_, rank_, s_q_, s_qn_, s_k_, s_pk_, s_ps_, s_ids_, s_sc_,
-                  s_pair_, s_pout_, s_qb_, s_qp_, s_ck_, s_cs_, s_sk_, s_ss_, s_beta_, s_fail_,
-                  s_err_, s_fq_, s_fid_, s_fsc_, s_fk_, s_sk2_, s_ss2_, s_beta2_, s_fail2_,
-                  s_cidx_, s_cq_, s_cid_, s_csc_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
+  void* bufs[] = {X_, nx2_, xn_, alive_, rank_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
                   sh_.rho_bits /* degen_count and degen live inside this block */, sh8_.X8,
                   sh8_.sx, sh8_.nx, sh8_.xn, sh8_.inv, sh8_.pen, sh8_.rho_bits};
+  for (auto& c : ctx_) {
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    void* cb[] = {c.s_q_, c.s_qn_, c.s_k_, c.s_pk_, c.s_ps_, c.s_ids_, c.s_sc_, c.s_pair_,
+                  c.s_pout_, c.s_qb_, c.s_qp_, c.s_ck_, c.s_cs_, c.s_sk_, c.s_ss_, c.s_beta_,
+                  c.s_fail_, c.s_err_, c.s_fq_, c.s_fid_, c.s_fsc_, c.s_fk_, c.s_sk2_, c.s_ss2_,
+                  c.s_beta2_, c.s_fail2_, c.s_cidx_, c.s_cq_, c.s_cid_, c.s_csc_};
+    for (void* p : cb)
+      if (p) (void)hipFree(p);
+    if (c.h_fail_) (void)hipHostFree(c.h_fail_);
+    if (c.h_io_) (void)hipHostFree(c.h_io_);
+    if (c.h_stage_) (void)hipHostFree(c.h_stage_);
+    for (auto& ps : c.prof_pending_) {
+      (void)hipEventDestroy(ps.a);
+      (void)hipEventDestroy(ps.b);
+    }
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+  }
   for (void* p : bufs)
     if (p) (void)hipFree(p);
-  if (h_fail_) (void)hipHostFree(h_fail_);
-  if (h_io_) (void)hipHostFree(h_io_);
   if (h_stage_) (void)hipHostFree(h_stage_);
   for (auto& e : ev_pool_) (void)hipEventDestroy(e);
-  for (auto& ps : prof_pending_) {
-    (void)hipEventDestroy(ps.a);
-    (void)hipEventDestroy(ps.b);
-  }
   (void)hipStreamDestroy(stream_);
 }
 
@@ -174,9 +196,11 @@ void Engine::dev_free(void* p, size_t bytes) {
   dev_bytes_ -= std::min(bytes, dev_bytes_);
 }
 
+// Grow-only device scratch of the current search context: a buffer being
+// replaced may still be read by that context's queued kernels.
 int Engine::scratch(void** p, size_t* have, size_t need) {
   if (need <= *have && *p) return SVDB_OK;
-  HIPCHK(hipStreamSynchronize(stream_), "sync");
+  HIPCHK(hipStreamSynchronize(c_->stream), "sync");
   dev_free(*p, *have);
   *p = nullptr;
   *have = 0;
@@ -196,20 +220,20 @@ int Engine::scratch(void** p, size_t* have, size_t need) {
 // uses pageable vectors instead.
 static constexpr size_t kMaxStage = 256ull << 20;
 
-void* Engine::host_stage(size_t bytes) {
+void* Engine::host_stage(void** buf, size_t* have, size_t bytes) {
   if (bytes > kMaxStage) return nullptr;
-  if (h_stage_ && h_stage_n_ >= bytes) return h_stage_;
-  if (h_stage_) (void)hipHostFree(h_stage_);
-  h_stage_ = nullptr;
-  h_stage_n_ = 0;
+  if (*buf && *have >= bytes) return *buf;
+  if (*buf) (void)hipHostFree(*buf);
+  *buf = nullptr;
+  *have = 0;
   const size_t n = round_up(std::max<size_t>(bytes, 1u << 20), 1u << 20);
-  if (hipHostMalloc(&h_stage_, n) != hipSuccess) {
-    h_stage_ = nullptr;
+  if (hipHostMalloc(buf, n) != hipSuccess) {
+    *buf = nullptr;
     (void)hipGetLastError();
     return nullptr;
   }
-  h_stage_n_ = n;
-  return h_stage_;
+  *have = n;
+  return *buf;
 }
 
 int Engine::ensure_dim(size_t d) {
@@ -403,6 +427,7 @@ int Engine::alloc_shadow8(size_t nc) {
 
 int Engine::enable_shadow(int precision) {
   std::lock_guard<std::mutex> g(mu_);
+  if (int rc0 = complete_pending()) return rc0;
   if (precision != SVDB_PREC_BF16 && precision != SVDB_PREC_FP8 && precision != SVDB_PREC_AUTO) {
     set_error(SVDB_EINVAL, "enable_shadow takes SVDB_PREC_BF16, SVDB_PREC_FP8 or SVDB_PREC_AUTO");
     return SVDB_EINVAL;
@@ -457,9 +482,9 @@ int Engine::sync_alive() {
   }
   const size_t w0 = dirty_word_, n = nw - w0;
   HIPCHK(hipMemcpyAsync(alive_ + w0, h_alive_.data() + w0, n * sizeof(uint64_t),
-                        hipMemcpyHostToDevice, stream_), "alive upload");
+                        hipMemcpyHostToDevice, c_->stream), "alive upload");
   HIPCHK(hipMemcpyAsync(rank_ + w0, h_rank_.data() + w0, n * sizeof(uint32_t),
-                        hipMemcpyHostToDevice, stream_), "rank upload");
+                        hipMemcpyHostToDevice, c_->stream), "rank upload");
   dirty_word_ = SIZE_MAX;
   return SVDB_OK;
 }
@@ -471,10 +496,10 @@ int Engine::refresh_degen() {
   if (!degen_dirty_) return SVDB_OK;
   if (lowp_)
     HIPCHK(launch_rebuild_degen(sh_.pen, alive_, (uint32_t)nphys_, sh_.degen_count, sh_.degen,
-                                stream_), "degenerate list");
+                                c_->stream), "degenerate list");
   if (lowp8_)
     HIPCHK(launch_rebuild_degen(sh8_.pen, alive_, (uint32_t)nphys_, sh8_.degen_count, sh8_.degen,
-                                stream_), "degenerate list");
+                                c_->stream), "degenerate list");
   degen_dirty_ = false;
   return SVDB_OK;
 }
@@ -504,6 +529,9 @@ int Engine::append(const double* rows, size_t n, size_t lds, size_t dim, uint32_
   std::lock_guard<std::mutex> g(mu_);
   if (!ok_) return SVDB_EDEVICE;
   if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  // searches in flight were submitted before this mutation: they finish
+  // against the store as it was
+  if (int rc0 = complete_pending()) return rc0;
   int rc = ensure_dim(dim);
   if (rc) return rc;
   if ((rc = ensure_capacity(nphys_ + n))) return rc;
@@ -513,7 +541,7 @@ int Engine::append(const double* rows, size_t n, size_t lds, size_t dim, uint32_
     // the device layout (stride ld, zero padding), one DMA per chunk.
     const size_t row_b = ld_ * sizeof(double);
     const size_t chunk = std::max<size_t>(1, std::min<size_t>(n, (64u << 20) / row_b));
-    double* st = static_cast<double*>(host_stage(chunk * row_b));
+    double* st = static_cast<double*>(host_stage(&h_stage_, &h_stage_n_, chunk * row_b));
     if (!st) {
       set_error(SVDB_EDEVICE, "hipHostMalloc failed");
       return SVDB_EDEVICE;
@@ -548,6 +576,9 @@ int Engine::overwrite(size_t logical, const double* row, size_t dim) {
   std::lock_guard<std::mutex> g(mu_);
   if (!ok_) return SVDB_EDEVICE;
   if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  // searches in flight were submitted before this mutation: they finish
+  // against the store as it was
+  if (int rc0 = complete_pending()) return rc0;
   if (logical >= nalive_) {
     set_error(SVDB_EINVAL, "index out of range");
     return SVDB_EINVAL;
@@ -572,6 +603,9 @@ int Engine::erase(size_t logical) {
   std::lock_guard<std::mutex> g(mu_);
   if (!ok_) return SVDB_EDEVICE;
   if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  // searches in flight were submitted before this mutation: they finish
+  // against the store as it was
+  if (int rc0 = complete_pending()) return rc0;
   if (logical >= nalive_) {
     set_error(SVDB_EINVAL, "index out of range");
     return SVDB_EINVAL;
@@ -711,21 +745,21 @@ hipEvent_t Engine::take_event() {
 void Engine::prof_begin(int which) {
   if (!prof_ || prof_mute_) return;
   ProfSlot ps{which, take_event(), take_event()};
-  (void)hipEventRecord(ps.a, stream_);
-  prof_pending_.push_back(ps);
+  (void)hipEventRecord(ps.a, c_->stream);
+  c_->prof_pending_.push_back(ps);
 }
 
 void Engine::prof_end(int which) {
   if (!prof_ || prof_mute_) return;
-  for (auto it = prof_pending_.rbegin(); it != prof_pending_.rend(); ++it)
+  for (auto it = c_->prof_pending_.rbegin(); it != c_->prof_pending_.rend(); ++it)
     if (it->kind == which) {
-      (void)hipEventRecord(it->b, stream_);
+      (void)hipEventRecord(it->b, c_->stream);
       return;
     }
 }
 
 void Engine::prof_collect() {
-  for (auto& ps : prof_pending_) {
+  for (auto& ps : c_->prof_pending_) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ps.a, ps.b) == hipSuccess) {
       kt_[ps.kind].launches++;
@@ -734,7 +768,7 @@ void Engine::prof_collect() {
     ev_pool_.push_back(ps.a);
     ev_pool_.push_back(ps.b);
   }
-  prof_pending_.clear();
+  c_->prof_pending_.clear();
 }
 
 int Engine::set_profiling(bool on) {
@@ -777,49 +811,68 @@ static constexpr size_t kMaxBatch = 8192;
 // Queries per launch of the low-precision pipeline.
 static constexpr size_t kMaxBatchLowp = 16384;
 
+// Row slices of an exact scan of nq queries (a multiple of 8, one set per
+// XCD; 0 for an empty store), with at most max_entries partial-list entries
+// (nq × slices × waves × kmax); 0 with rows present when even 8 slices
+// would need more.
+uint32_t Engine::exact_slices(uint32_t nq, int kmax, size_t max_entries) const {
+  if (nphys_ == 0) return 0;
+  // About 8,192 blocks (query groups × row slices): short slices keep the
+  // query groups of one slice, which run back to back on one XCD, close
+  // enough together for that XCD's L2 to serve them all the slice's rows
+  // (1M × 128, 1,024 queries: 15.6 → 12.7 ms Euclidean, 12.0 → 9.9 ms dot
+  // against 1,024 blocks; 16,384 lost again in the merge).  At most 1,024
+  // slices, so a few queries do not leave the merge thousands of lists.
+  const size_t groups = (nq + kScanTQ - 1) / kScanTQ;
+  size_t want = std::min<size_t>(1024, (8192 + groups - 1) / groups);
+  size_t maxs = std::max<size_t>(1, nphys_ / 1024);
+  size_t nslices = std::max<size_t>(1, std::min(want, maxs));
+  // large batches at large k take fewer, longer slices
+  while (nslices > 8 && (size_t)nq * nslices * kWavesPerBlock * kmax > max_entries)
+    nslices = (nslices + 1) / 2;
+  nslices = round_up(nslices, 8);
+  if ((size_t)nq * nslices * kWavesPerBlock * kmax > max_entries) return 0;
+  return (uint32_t)nslices;
+}
+
 int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk,
                       int metric, int kmax, uint64_t* dids, double* dscores, uint32_t ostride) {
   int rc;
   if (nq == 0) return SVDB_OK;
+  // partial lists stay under 2^25 entries (~400 MB)
+  const uint32_t nslices = exact_slices(nq, kmax, std::max<size_t>((size_t)1 << 25,
+                                                                    (size_t)nq * 32 * kmax));
+  const size_t nlist = (size_t)nq * nslices * kWavesPerBlock * kmax;
+  if ((rc = scratch(&c_->s_pk_, &c_->s_pk_n_, std::max<size_t>(nlist, 1) * sizeof(uint64_t)))) return rc;
+  if ((rc = scratch(&c_->s_ps_, &c_->s_ps_n_, std::max<size_t>(nlist, 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&c_->s_qn_, &c_->s_qn_n_, (size_t)nq * sizeof(double)))) return rc;
+  return run_exact_with(dQ, ldq, nq, D, dk, metric, kmax, dids, dscores, ostride, nslices,
+                        (uint64_t*)c_->s_pk_, (uint32_t*)c_->s_ps_, (double*)c_->s_qn_);
+}
+
+// The exact scan and merge into caller-provided scratch: partial lists
+// pk / ps of nq × nslices × kWavesPerBlock × kmax entries, query norms qn.
+// Allocates nothing.
+int Engine::run_exact_with(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk,
+                           int metric, int kmax, uint64_t* dids, double* dscores,
+                           uint32_t ostride, uint32_t nslices, uint64_t* pk, uint32_t* ps,
+                           double* qn) {
+  int rc;
+  if (nq == 0) return SVDB_OK;
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
-  uint32_t nslices = 0, rps = 0;
-  if (nphys_ > 0) {
-    // About 8,192 blocks (query groups × row slices): short slices keep the
-    // query groups of one slice, which run back to back on one XCD, close
-    // enough together for that XCD's L2 to serve them all the slice's rows
-    // (1M × 128, 1,024 queries: 15.6 → 12.7 ms Euclidean, 12.0 → 9.9 ms dot
-    // against 1,024 blocks; 16,384 lost again in the merge).  At most 1,024
-    // slices, so a few queries do not leave the merge thousands of lists.
-    const size_t groups = (nq + kScanTQ - 1) / kScanTQ;
-    size_t want = std::min<size_t>(1024, (8192 + groups - 1) / groups);
-    size_t maxs = std::max<size_t>(1, nphys_ / 1024);
-    nslices = (uint32_t)std::max<size_t>(1, std::min(want, maxs));
-    // partial lists (nq × slices × waves × kmax entries of 12 bytes) stay
-    // under 2^25 entries (~400 MB): large batches at large k take fewer,
-    // longer slices
-    while (nslices > 8 && (size_t)nq * nslices * kWavesPerBlock * kmax > ((size_t)1 << 25))
-      nslices = (nslices + 1) / 2;
-    nslices = (uint32_t)round_up(nslices, 8);  // one set of slices per XCD
-    rps = (uint32_t)((nphys_ + nslices - 1) / nslices);
-  }
+  const uint32_t rps = nslices ? (uint32_t)((nphys_ + nslices - 1) / nslices) : 0;
   const uint32_t P = nslices * kWavesPerBlock;
-  const size_t nlist = (size_t)nq * P * kmax;
-  if ((rc = scratch(&s_pk_, &s_pk_n_, std::max<size_t>(nlist, 1) * sizeof(uint64_t)))) return rc;
-  if ((rc = scratch(&s_ps_, &s_ps_n_, std::max<size_t>(nlist, 1) * sizeof(uint32_t)))) return rc;
-  if ((rc = scratch(&s_qn_, &s_qn_n_, (size_t)nq * sizeof(double)))) return rc;
   if ((rc = sync_alive())) return rc;
-  if (metric == kCos) HIPCHK(launch_query_norms(dQ, ldq, nq, D, (double*)s_qn_, stream_), "k_query_norms");
+  if (metric == kCos) HIPCHK(launch_query_norms(dQ, ldq, nq, D, qn, c_->stream), "k_query_norms");
   if (nslices) {
-    ScanF64Args a{sv, dQ, ldq, (const double*)s_qn_, dk, nq, D, rps, P, (uint32_t)kmax,
-                  (uint64_t*)s_pk_, (uint32_t*)s_ps_};
+    ScanF64Args a{sv, dQ, ldq, qn, dk, nq, D, rps, P, (uint32_t)kmax, pk, ps};
     prof_begin(SVDB_K_SCAN_FP64);
-    HIPCHK(launch_scan_f64(a, metric, kmax, nslices, stream_), "k_scan_f64");
+    HIPCHK(launch_scan_f64(a, metric, kmax, nslices, c_->stream), "k_scan_f64");
     prof_end(SVDB_K_SCAN_FP64);
   }
-  MergeArgs m{sv, dk, nq, P, (uint32_t)kmax, (const uint64_t*)s_pk_, (const uint32_t*)s_ps_,
-              ostride, dids, dscores};
+  MergeArgs m{sv, dk, nq, P, (uint32_t)kmax, pk, ps, ostride, dids, dscores};
   prof_begin(SVDB_K_MERGE);
-  HIPCHK(launch_merge(m, metric, kmax, stream_), "k_merge");
+  HIPCHK(launch_merge(m, metric, kmax, c_->stream), "k_merge");
   prof_end(SVDB_K_MERGE);
   return SVDB_OK;
 }
@@ -827,6 +880,15 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
 // Low-precision first pass + shortlist + FP64 rerank for nq device queries
 // (all columns, one metric and k).  Queries the rerank cannot certify are
 // gathered and answered by the FP64 scan.
+// Shortlist: the configured width (64 by default) for small k; for BF16
+// 256 once k passes 32, where the k-th and L-th neighbours need more room
+// than the first-pass error bound leaves in a 2k-wide list.  FP8 starts at
+// the configured width (or k) and relies on the wider retries.
+uint32_t Engine::lowp_width(int kk, bool fp8, size_t shortlist) {
+  return (uint32_t)std::min<size_t>(
+      kMaxShort, std::max<size_t>({shortlist, (size_t)kk, (!fp8 && kk > 32) ? (size_t)256 : 0}));
+}
+
 int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k,
                      uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
   LowpRange r{metric, 0, nq, k, 0, 0, 0, 0, 0};
@@ -842,9 +904,14 @@ int Engine::run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int 
 // per range; the shortlists and the reranks of all ranges share a launch;
 // widening and fallback run per range, each with its own k and shortlist
 // width (k is the largest over ranges).
-int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, int k,
-                            uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
+int Engine::lowp_enqueue(const double* dQ, size_t ldq, LowpRange* R, int nr, int k,
+                         uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
   int rc;
+  if (nr < 1 || nr > 3) {
+    set_error(SVDB_EINVAL, "one to three query ranges");
+    return SVDB_EINVAL;
+  }
+  const size_t shortlist = shortlist_;
   const bool mixed = nr > 1;  // several ranges: slots aligned per range
   bool multi_metric = false;  // ... of different metrics: the mixed-metric scan
   for (int i = 1; i < nr; i++) multi_metric |= R[i].metric != R[0].metric;
@@ -868,15 +935,7 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   const size_t qbytes = fp8 ? 1 : sizeof(uint16_t);  // bytes per query element
   const uint32_t qpb = (uint32_t)(fp8 ? fp8_queries_per_block(dp) : lowp_queries_per_block(dp));
   const uint32_t nq_pad = (uint32_t)round_up(nq, qpb);
-  // Shortlist: the configured width (64 by default) for small k; for BF16
-  // 256 once k passes 32, where the k-th and L-th neighbours need more room
-  // than the first-pass error bound leaves in a 2k-wide list.  FP8 starts at
-  // the configured width (or k) and relies on the wider retries below.
-  auto width = [&](int kk) {
-    return (uint32_t)std::min<size_t>(
-        kMaxShort,
-        std::max<size_t>({shortlist_, (size_t)kk, (!fp8 && kk > 32) ? (size_t)256 : 0}));
-  };
+  auto width = [&](int kk) { return lowp_width(kk, fp8, shortlist); };
   const uint32_t L = width(k);  // the widest range's
   const uint32_t nrows_pad = (uint32_t)round_up(nphys_, kLowpTile);
   const uint32_t groups = nq_pad / qpb;
@@ -963,44 +1022,44 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   const size_t ncand = std::max(cand_end, pre ? (size_t)nq * NL_pre : (size_t)0);
   // query rows: nq_pad slots plus one block, since a launch over one range
   // (at slot offset soff) reads whole blocks of rows from soff on
-  if ((rc = scratch(&s_qb_, &s_qb_n_, ((size_t)nq_pad + qpb) * dp * qbytes))) return rc;
-  if ((rc = scratch(&s_qp_, &s_qp_n_, (size_t)nq_pad * sizeof(QParams)))) return rc;
-  if ((rc = scratch(&s_ck_, &s_ck_n_, ncand * sizeof(float)))) return rc;
-  if ((rc = scratch(&s_cs_, &s_cs_n_, ncand * sizeof(uint32_t)))) return rc;
-  if ((rc = scratch(&s_sk_, &s_sk_n_, (size_t)nq * L * sizeof(float)))) return rc;
-  if ((rc = scratch(&s_ss_, &s_ss_n_, (size_t)nq * L * sizeof(uint32_t)))) return rc;
-  if ((rc = scratch(&s_beta_, &s_beta_n_, (size_t)nq * sizeof(float)))) return rc;
+  if ((rc = scratch(&c_->s_qb_, &c_->s_qb_n_, ((size_t)nq_pad + qpb) * dp * qbytes))) return rc;
+  if ((rc = scratch(&c_->s_qp_, &c_->s_qp_n_, (size_t)nq_pad * sizeof(QParams)))) return rc;
+  if ((rc = scratch(&c_->s_ck_, &c_->s_ck_n_, ncand * sizeof(float)))) return rc;
+  if ((rc = scratch(&c_->s_cs_, &c_->s_cs_n_, ncand * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&c_->s_sk_, &c_->s_sk_n_, (size_t)nq * L * sizeof(float)))) return rc;
+  if ((rc = scratch(&c_->s_ss_, &c_->s_ss_n_, (size_t)nq * L * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&c_->s_beta_, &c_->s_beta_n_, (size_t)nq * sizeof(float)))) return rc;
   // fail counts (one per range) then each range's fail list at nr + soff
-  if ((rc = scratch(&s_fail_, &s_fail_n_, ((size_t)nq + nr) * sizeof(uint32_t)))) return rc;
-  if ((rc = scratch(&s_err_, &s_err_n_, (size_t)nq * sizeof(double)))) return rc;
-  if (!h_fail_) HIPCHK(hipHostMalloc((void**)&h_fail_, sizeof(uint32_t)), "hipHostMalloc");
+  if ((rc = scratch(&c_->s_fail_, &c_->s_fail_n_, ((size_t)nq + nr) * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&c_->s_err_, &c_->s_err_n_, (size_t)nq * sizeof(double)))) return rc;
+  if (!c_->h_fail_) HIPCHK(hipHostMalloc((void**)&c_->h_fail_, sizeof(uint32_t)), "hipHostMalloc");
   const size_t io_b = round_up(nr * sizeof(uint32_t), 64) + (size_t)nq * sizeof(double);
-  if (h_io_n_ < io_b) {
-    if (h_io_) (void)hipHostFree(h_io_);
-    h_io_ = nullptr;
-    h_io_n_ = 0;
-    HIPCHK(hipHostMalloc(&h_io_, round_up(io_b, 4096)), "hipHostMalloc");
-    h_io_n_ = round_up(io_b, 4096);
+  if (c_->h_io_n_ < io_b) {
+    if (c_->h_io_) (void)hipHostFree(c_->h_io_);
+    c_->h_io_ = nullptr;
+    c_->h_io_n_ = 0;
+    HIPCHK(hipHostMalloc(&c_->h_io_, round_up(io_b, 4096)), "hipHostMalloc");
+    c_->h_io_n_ = round_up(io_b, 4096);
   }
-  uint32_t* h_counts = static_cast<uint32_t*>(h_io_);
-  double* h_err = reinterpret_cast<double*>(static_cast<char*>(h_io_) + round_up(nr * sizeof(uint32_t), 64));
+  uint32_t* h_counts = static_cast<uint32_t*>(c_->h_io_);
+  double* h_err = reinterpret_cast<double*>(static_cast<char*>(c_->h_io_) + round_up(nr * sizeof(uint32_t), 64));
   if ((rc = sync_alive())) return rc;
   if ((rc = refresh_degen())) return rc;
 
-  HIPCHK(hipMemsetAsync(s_err_, 0, (size_t)nq * sizeof(double), stream_), "memset");
+  HIPCHK(hipMemsetAsync(c_->s_err_, 0, (size_t)nq * sizeof(double), c_->stream), "memset");
   // query prep per range; each range's slots run to the next range (the last
   // to nq_pad), the slots past its queries flagged as padding
   for (int i = 0; i < nr; i++) {
     const uint32_t send = i + 1 < nr ? R[i + 1].soff : nq_pad;
     const double* q = dQ + (size_t)R[i].qoff * ldq;
-    unsigned char* qb = (unsigned char*)s_qb_ + (size_t)R[i].soff * dp * qbytes;
-    QParams* qp = (QParams*)s_qp_ + R[i].soff;
+    unsigned char* qb = (unsigned char*)c_->s_qb_ + (size_t)R[i].soff * dp * qbytes;
+    QParams* qp = (QParams*)c_->s_qp_ + R[i].soff;
     if (fp8)
       HIPCHK(launch_query_prep8(q, ldq, R[i].n, send - R[i].soff, (uint32_t)d_, dp, R[i].metric,
-                                sh8_.rho_bits, (uint8_t*)qb, qp, stream_), "k_query_prep8");
+                                sh8_.rho_bits, (uint8_t*)qb, qp, c_->stream), "k_query_prep8");
     else
       HIPCHK(launch_query_prep(q, ldq, R[i].n, send - R[i].soff, (uint32_t)d_, dp, R[i].metric,
-                               sh_.rho_bits, (uint16_t*)qb, qp, stream_), "k_query_prep");
+                               sh_.rho_bits, (uint16_t*)qb, qp, c_->stream), "k_query_prep");
   }
   // per-row ordering-key arrays: BF16 L2 (nx, pen), dot (xn, pen), cosine
   // (inv, pen); FP8 L2 (nx, sx), dot (xn, sx), cosine (sx/‖x‖, pen)
@@ -1012,13 +1071,13 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   const uint32_t* degen = fp8 ? sh8_.degen : sh_.degen;
   const uint32_t* degen_count = fp8 ? sh8_.degen_count : sh_.degen_count;
   auto scan = [&](const LowpScanArgs& x, int m, uint32_t nsl, bool minonly) {
-    return fp8 ? launch_scan_fp8(x, m, nsl, minonly, stream_)
-               : launch_scan_bf16(x, m, nsl, minonly, stream_);
+    return fp8 ? launch_scan_fp8(x, m, nsl, minonly, c_->stream)
+               : launch_scan_bf16(x, m, nsl, minonly, c_->stream);
   };
   // the fields every launch shares; each sets its own rows, queries and lists
   LowpScanArgs sa{fp8 ? (const uint16_t*)sh8_.X8 : sh_.Xb, dp, nullptr, nullptr, nullptr,
-                  nullptr, (const uint16_t*)s_qb_, (const QParams*)s_qp_, nq, 0, 0,
-                  nrows_pad, 0, (float*)s_ck_, (uint32_t*)s_cs_, nullptr, 0};
+                  nullptr, (const uint16_t*)c_->s_qb_, (const QParams*)c_->s_qp_, nq, 0, 0,
+                  nrows_pad, 0, (float*)c_->s_ck_, (uint32_t*)c_->s_cs_, nullptr, 0};
   if (pre) {
     prof_begin(SVDB_K_PREPASS);
     for (int i = 0; i < nr; i++) {
@@ -1026,21 +1085,21 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
       pa.w0 = w0_of(R[i].metric);
       pa.w1 = w1_of(R[i].metric);
       pa.w2 = pa.w3 = nullptr;
-      pa.Qb = (const uint16_t*)((const unsigned char*)s_qb_ + (size_t)R[i].soff * dp * qbytes);
-      pa.qp = (const QParams*)s_qp_ + R[i].soff;
+      pa.Qb = (const uint16_t*)((const unsigned char*)c_->s_qb_ + (size_t)R[i].soff * dp * qbytes);
+      pa.qp = (const QParams*)c_->s_qp_ + R[i].soff;
       pa.nq = R[i].n;
       pa.rows_per_slice = tp * kLowpTile;
       pa.slice_stride = (tiles / ns_pre) * kLowpTile;
       pa.NL = NL_pre;
-      pa.cand_key = (float*)s_ck_ + (size_t)R[i].soff * NL_pre;
+      pa.cand_key = (float*)c_->s_ck_ + (size_t)R[i].soff * NL_pre;
       HIPCHK(scan(pa, R[i].metric, ns_pre, true), "first-pass scan (sample)");
-      // thresholds in s_beta_, read by the main scan before the shortlist
+      // thresholds in c_->s_beta_, read by the main scan before the shortlist
       // below overwrites them (padding slots' values are never used)
-      HIPCHK(launch_kth_key((const float*)s_ck_ + (size_t)R[i].soff * NL_pre, R[i].n, NL_pre,
-                            width(R[i].k), (float*)s_beta_ + R[i].soff, stream_), "k_kth_key");
+      HIPCHK(launch_kth_key((const float*)c_->s_ck_ + (size_t)R[i].soff * NL_pre, R[i].n, NL_pre,
+                            width(R[i].k), (float*)c_->s_beta_ + R[i].soff, c_->stream), "k_kth_key");
     }
     prof_end(SVDB_K_PREPASS);
-    sa.tau0 = (const float*)s_beta_;
+    sa.tau0 = (const float*)c_->s_beta_;
     sa.tau0_stride = 1;
   }
   prof_begin(SVDB_K_SCAN_LOWP);
@@ -1053,13 +1112,13 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     ma.w1 = mk ? sh_.pen : w1_of(g.metric);
     ma.w2 = mk ? sh_.xn : nullptr;
     ma.w3 = mk ? sh_.inv : nullptr;
-    ma.Qb = (const uint16_t*)((const unsigned char*)s_qb_ + (size_t)g.soff * dp * qbytes);
-    ma.qp = (const QParams*)s_qp_ + g.soff;
+    ma.Qb = (const uint16_t*)((const unsigned char*)c_->s_qb_ + (size_t)g.soff * dp * qbytes);
+    ma.qp = (const QParams*)c_->s_qp_ + g.soff;
     ma.nq = g.n;
     ma.rows_per_slice = ma.slice_stride = r.rps;
     ma.NL = r.NL;
-    ma.cand_key = (float*)s_ck_ + r.coff;
-    ma.cand_slot = (uint32_t*)s_cs_ + r.coff;
+    ma.cand_key = (float*)c_->s_ck_ + r.coff;
+    ma.cand_slot = (uint32_t*)c_->s_cs_ + r.coff;
     if (ma.tau0) ma.tau0 += g.soff;
     HIPCHK(scan(ma, g.metric, r.ns, false), "first-pass scan");
   }
@@ -1069,30 +1128,30 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
   // every range's shortlist, in as few launches as the kernels allow
   std::vector<ShortlistArgs> sls(nr);
   for (int i = 0; i < nr; i++)
-    sls[i] = ShortlistArgs{(const float*)s_ck_ + R[i].coff, (const uint32_t*)s_cs_ + R[i].coff,
+    sls[i] = ShortlistArgs{(const float*)c_->s_ck_ + R[i].coff, (const uint32_t*)c_->s_cs_ + R[i].coff,
                            R[i].n, R[i].NL, width(R[i].k),
-                           (uint32_t*)s_ss_ + (size_t)R[i].soff * L,
-                           (float*)s_sk_ + (size_t)R[i].soff * L, (float*)s_beta_ + R[i].soff};
+                           (uint32_t*)c_->s_ss_ + (size_t)R[i].soff * L,
+                           (float*)c_->s_sk_ + (size_t)R[i].soff * L, (float*)c_->s_beta_ + R[i].soff};
   prof_begin(SVDB_K_SHORTLIST);
-  HIPCHK(launch_shortlist_ranges(sls.data(), nr, stream_), "k_shortlist");
+  HIPCHK(launch_shortlist_ranges(sls.data(), nr, c_->stream), "k_shortlist");
   prof_end(SVDB_K_SHORTLIST);
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
   const uint32_t Lmax = L;  // shortlist block stride per slot
   // FP64 rerank of every range's shortlist; fail counts and the first-pass
   // error report come back together after one wait
-  HIPCHK(hipMemsetAsync(s_fail_, 0, nr * sizeof(uint32_t), stream_), "memset");
+  HIPCHK(hipMemsetAsync(c_->s_fail_, 0, nr * sizeof(uint32_t), c_->stream), "memset");
   auto rerank_args = [&](int ri) {
     const LowpRange& rg = R[ri];
-    const uint32_t* fl = (uint32_t*)s_fail_ + nr + rg.soff;
+    const uint32_t* fl = (uint32_t*)c_->s_fail_ + nr + rg.soff;
     return RerankArgs{sv, dQ + (size_t)rg.qoff * ldq, ldq, (uint32_t)d_,
-                      (const QParams*)s_qp_ + rg.soff,
-                      (const uint32_t*)s_ss_ + (size_t)rg.soff * Lmax,
-                      (const float*)s_sk_ + (size_t)rg.soff * Lmax,
-                      (const float*)s_beta_ + rg.soff, degen, degen_count, w0_of(rg.metric),
+                      (const QParams*)c_->s_qp_ + rg.soff,
+                      (const uint32_t*)c_->s_ss_ + (size_t)rg.soff * Lmax,
+                      (const float*)c_->s_sk_ + (size_t)rg.soff * Lmax,
+                      (const float*)c_->s_beta_ + rg.soff, degen, degen_count, w0_of(rg.metric),
                       w1_of(rg.metric), rg.n, width(rg.k), (uint32_t)rg.k, ostride,
                       dids + (size_t)rg.qoff * ostride, dscores + (size_t)rg.qoff * ostride,
-                      (uint32_t*)s_fail_ + ri, const_cast<uint32_t*>(fl),
-                      (double*)s_err_ + rg.soff};
+                      (uint32_t*)c_->s_fail_ + ri, const_cast<uint32_t*>(fl),
+                      (double*)c_->s_err_ + rg.soff};
   };
   std::vector<RerankArgs> ras(nr);
   std::vector<int> rms(nr);
@@ -1101,13 +1160,74 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     rms[ri] = R[ri].metric;
   }
   prof_begin(SVDB_K_RERANK);
-  HIPCHK(launch_rerank_ranges(ras.data(), rms.data(), nr, stream_), "k_rerank");
+  HIPCHK(launch_rerank_ranges(ras.data(), rms.data(), nr, c_->stream), "k_rerank");
   prof_end(SVDB_K_RERANK);
-  HIPCHK(hipMemcpyAsync(h_counts, s_fail_, nr * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+  HIPCHK(hipMemcpyAsync(h_counts, c_->s_fail_, nr * sizeof(uint32_t), hipMemcpyDeviceToHost, c_->stream),
          "copy");
-  HIPCHK(hipMemcpyAsync(h_err, s_err_, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost, stream_),
+  HIPCHK(hipMemcpyAsync(h_err, c_->s_err_, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost, c_->stream),
          "copy");
-  HIPCHK(hipStreamSynchronize(stream_), "lowp");
+  // what lowp_complete needs once the counts are back
+  SearchCtx::LowpPending& lp = c_->lp;
+  lp.on = true;
+  for (int i = 0; i < nr; i++) lp.R[i] = R[i];
+  lp.nr = nr;
+  lp.k = k;
+  lp.dQ = dQ;
+  lp.ldq = ldq;
+  lp.dids = dids;
+  lp.dscores = dscores;
+  lp.ostride = ostride;
+  lp.fp8 = fp8;
+  lp.nq = nq;
+  lp.shortlist = shortlist;
+  lp.failed = false;
+  return SVDB_OK;
+}
+
+// Waits for lowp_enqueue's work and answers what the first pass could not
+// certify: the widening retries, the FP8 queries' BF16 second pass, and the
+// FP64 rescan.
+int Engine::lowp_complete() {
+  int rc;
+  SearchCtx::LowpPending lp = c_->lp;  // a nested pass below reuses c_->lp
+  c_->lp.on = false;
+  if (!lp.on) return SVDB_OK;
+  LowpRange* R = lp.R;
+  const int nr = lp.nr;
+  const double* dQ = lp.dQ;
+  const size_t ldq = lp.ldq;
+  uint64_t* dids = lp.dids;
+  double* dscores = lp.dscores;
+  const uint32_t ostride = lp.ostride;
+  const bool fp8 = lp.fp8;
+  const uint32_t nq = lp.nq;
+  auto width = [&](int kk) { return lowp_width(kk, fp8, lp.shortlist); };
+  auto w0_of = [&](int m) {
+    return fp8 ? (m == kL2 ? sh8_.nx : m == kDot ? sh8_.xn : sh8_.inv)
+               : (m == kL2 ? sh_.nx : m == kDot ? sh_.xn : sh_.inv);
+  };
+  auto w1_of = [&](int m) { return fp8 ? (m == kCos ? sh8_.pen : sh8_.sx) : sh_.pen; };
+  const uint32_t* degen = fp8 ? sh8_.degen : sh_.degen;
+  const uint32_t* degen_count = fp8 ? sh8_.degen_count : sh_.degen_count;
+  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
+  const uint32_t Lmax = width(lp.k);  // shortlist block stride per slot
+  auto rerank_args = [&](int ri) {
+    const LowpRange& rg = R[ri];
+    const uint32_t* fl = (uint32_t*)c_->s_fail_ + nr + rg.soff;
+    return RerankArgs{sv, dQ + (size_t)rg.qoff * ldq, ldq, (uint32_t)d_,
+                      (const QParams*)c_->s_qp_ + rg.soff,
+                      (const uint32_t*)c_->s_ss_ + (size_t)rg.soff * Lmax,
+                      (const float*)c_->s_sk_ + (size_t)rg.soff * Lmax,
+                      (const float*)c_->s_beta_ + rg.soff, degen, degen_count, w0_of(rg.metric),
+                      w1_of(rg.metric), rg.n, width(rg.k), (uint32_t)rg.k, ostride,
+                      dids + (size_t)rg.qoff * ostride, dscores + (size_t)rg.qoff * ostride,
+                      (uint32_t*)c_->s_fail_ + ri, const_cast<uint32_t*>(fl),
+                      (double*)c_->s_err_ + rg.soff};
+  };
+  uint32_t* h_counts = static_cast<uint32_t*>(c_->h_io_);
+  double* h_err = reinterpret_cast<double*>(static_cast<char*>(c_->h_io_) +
+                                            round_up(nr * sizeof(uint32_t), 64));
+  HIPCHK(hipStreamSynchronize(c_->stream), "lowp");
   double mx = 0.0;
   for (uint32_t i = 0; i < nq; i++) mx = std::max(mx, h_err[i]);
   std::vector<uint32_t> nfail(h_counts, h_counts + nr);
@@ -1121,12 +1241,14 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     const uint32_t nf1 = nfail[ri];
     st_.certified += nqr - nf1;
     if (nf1 == 0) continue;
+    c_->lp.failed = true;  // (restored below for the caller's record)
+    lp.failed = true;
     // this range's views: slot-space arrays at soff, query-space at qoff
     const double* rQ = dQ + (size_t)rg.qoff * ldq;
     uint64_t* rids = dids + (size_t)rg.qoff * ostride;
     double* rsc = dscores + (size_t)rg.qoff * ostride;
-    const float* rck = (const float*)s_ck_ + rg.coff;
-    const uint32_t* rcs = (const uint32_t*)s_cs_ + rg.coff;
+    const float* rck = (const float*)c_->s_ck_ + rg.coff;
+    const uint32_t* rcs = (const uint32_t*)c_->s_cs_ + rg.coff;
     const RerankArgs ra = rerank_args(ri);
     uint32_t* fail_list = ra.fail_list;  // may be redirected to a widening retry's list
     uint32_t nf = nf1;
@@ -1136,45 +1258,45 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     // uncertified go on.
     for (uint32_t Lw : {1024u, (uint32_t)kMaxShort}) {
       if (nf == 0 || Lw <= L || (size_t)NL * kLowpLP < 2 * (size_t)Lw) continue;
-      if ((rc = scratch(&s_sk2_, &s_sk2_n_, (size_t)nf * Lw * sizeof(float)))) return rc;
-      if ((rc = scratch(&s_ss2_, &s_ss2_n_, (size_t)nf * Lw * sizeof(uint32_t)))) return rc;
-      if ((rc = scratch(&s_beta2_, &s_beta2_n_, (size_t)nf * sizeof(float)))) return rc;
-      if ((rc = scratch(&s_fail2_, &s_fail2_n_, 2 * ((size_t)nf + 1) * sizeof(uint32_t)))) return rc;
+      if ((rc = scratch(&c_->s_sk2_, &c_->s_sk2_n_, (size_t)nf * Lw * sizeof(float)))) return rc;
+      if ((rc = scratch(&c_->s_ss2_, &c_->s_ss2_n_, (size_t)nf * Lw * sizeof(uint32_t)))) return rc;
+      if ((rc = scratch(&c_->s_beta2_, &c_->s_beta2_n_, (size_t)nf * sizeof(float)))) return rc;
+      if ((rc = scratch(&c_->s_fail2_, &c_->s_fail2_n_, 2 * ((size_t)nf + 1) * sizeof(uint32_t)))) return rc;
       // two fail lists alternate: this stage reads `fail_list`, writes the other
       uint32_t* out_count =
-          (uint32_t*)s_fail2_ + (fail_list == (uint32_t*)s_fail2_ + 1 ? nf + 1 : 0);
+          (uint32_t*)c_->s_fail2_ + (fail_list == (uint32_t*)c_->s_fail2_ + 1 ? nf + 1 : 0);
       uint32_t* out_list = out_count + 1;
-      HIPCHK(hipMemsetAsync(out_count, 0, sizeof(uint32_t), stream_), "memset");
-      ShortlistArgs sw{rck, rcs, nf, NL, Lw, (uint32_t*)s_ss2_, (float*)s_sk2_,
-                       (float*)s_beta2_, fail_list};
+      HIPCHK(hipMemsetAsync(out_count, 0, sizeof(uint32_t), c_->stream), "memset");
+      ShortlistArgs sw{rck, rcs, nf, NL, Lw, (uint32_t*)c_->s_ss2_, (float*)c_->s_sk2_,
+                       (float*)c_->s_beta2_, fail_list};
       prof_begin(SVDB_K_SHORTLIST);
-      HIPCHK(launch_shortlist(sw, stream_), "k_shortlist (widened)");
+      HIPCHK(launch_shortlist(sw, c_->stream), "k_shortlist (widened)");
       prof_end(SVDB_K_SHORTLIST);
       RerankArgs rw = ra;
-      rw.short_slot = (const uint32_t*)s_ss2_;
-      rw.short_key = (const float*)s_sk2_;
-      rw.beta = (const float*)s_beta2_;
+      rw.short_slot = (const uint32_t*)c_->s_ss2_;
+      rw.short_key = (const float*)c_->s_sk2_;
+      rw.beta = (const float*)c_->s_beta2_;
       rw.nq = nf;
       rw.L = Lw;
       rw.fail_count = out_count;
       rw.fail_list = out_list;
       rw.qidx = fail_list;
       prof_begin(SVDB_K_RERANK);
-      HIPCHK(launch_rerank(rw, metric, stream_), "k_rerank (widened)");
+      HIPCHK(launch_rerank(rw, metric, c_->stream), "k_rerank (widened)");
       prof_end(SVDB_K_RERANK);
-      HIPCHK(hipMemcpyAsync(h_fail_, out_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+      HIPCHK(hipMemcpyAsync(c_->h_fail_, out_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c_->stream),
              "copy");
-      HIPCHK(hipStreamSynchronize(stream_), "lowp widened");
-      nf = *h_fail_;
+      HIPCHK(hipStreamSynchronize(c_->stream), "lowp widened");
+      nf = *c_->h_fail_;
       fail_list = out_list;
     }
     // the error report again, now with the widened queries certified
     if (nf != nf1) {
-      double* he = reinterpret_cast<double*>(static_cast<char*>(h_io_) +
+      double* he = reinterpret_cast<double*>(static_cast<char*>(c_->h_io_) +
                                              round_up(nr * sizeof(uint32_t), 64));
-      HIPCHK(hipMemcpyAsync(he, (double*)s_err_ + rg.soff, nqr * sizeof(double),
-                            hipMemcpyDeviceToHost, stream_), "copy");
-      HIPCHK(hipStreamSynchronize(stream_), "copy");
+      HIPCHK(hipMemcpyAsync(he, (double*)c_->s_err_ + rg.soff, nqr * sizeof(double),
+                            hipMemcpyDeviceToHost, c_->stream), "copy");
+      HIPCHK(hipStreamSynchronize(c_->stream), "copy");
       for (uint32_t i = 0; i < nqr; i++) mx = std::max(mx, he[i]);
     }
     st_.certified += nf1 - nf;
@@ -1185,51 +1307,64 @@ int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, 
     // 16x tighter); that pass rescans in FP64 only what it cannot certify.
     const bool bf16_ok = lowp_ && (sh_.dp <= 256 || bf16_wide_dp_supported(sh_.dp));
     if (fp8 && bf16_ok) {
-      if ((rc = scratch(&s_cidx_, &s_cidx_n_, (size_t)nf * sizeof(uint32_t)))) return rc;
-      if ((rc = scratch(&s_cq_, &s_cq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
-      if ((rc = scratch(&s_cid_, &s_cid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
-      if ((rc = scratch(&s_csc_, &s_csc_n_, (size_t)nf * k * sizeof(double)))) return rc;
+      if ((rc = scratch(&c_->s_cidx_, &c_->s_cidx_n_, (size_t)nf * sizeof(uint32_t)))) return rc;
+      if ((rc = scratch(&c_->s_cq_, &c_->s_cq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
+      if ((rc = scratch(&c_->s_cid_, &c_->s_cid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
+      if ((rc = scratch(&c_->s_csc_, &c_->s_csc_n_, (size_t)nf * k * sizeof(double)))) return rc;
       // the BF16 pass reuses the fail-list buffers: keep this list apart
-      HIPCHK(hipMemcpyAsync(s_cidx_, fail_list, nf * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                            stream_), "copy");
-      HIPCHK(launch_gather_rows(rQ, ldq, (const uint32_t*)s_cidx_, nf, (uint32_t)d_,
-                                (double*)s_cq_, d_, stream_), "k_gather_rows");
+      HIPCHK(hipMemcpyAsync(c_->s_cidx_, fail_list, nf * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                            c_->stream), "copy");
+      HIPCHK(launch_gather_rows(rQ, ldq, (const uint32_t*)c_->s_cidx_, nf, (uint32_t)d_,
+                                (double*)c_->s_cq_, d_, c_->stream), "k_gather_rows");
       st_.second_pass += nf;
       prof_begin(SVDB_K_SECOND_PASS);
       prof_mute_ = true;
       const double keep = mx;
-      rc = run_lowp((const double*)s_cq_, d_, nf, metric, k, (uint64_t*)s_cid_, (double*)s_csc_,
+      rc = run_lowp((const double*)c_->s_cq_, d_, nf, metric, k, (uint64_t*)c_->s_cid_, (double*)c_->s_csc_,
                     (uint32_t)k, false);
       prof_mute_ = false;
       if (rc) return rc;
       prof_end(SVDB_K_SECOND_PASS);
       mx = std::max(keep, st_.last_max_rel_err);
-      HIPCHK(launch_scatter_results((const uint64_t*)s_cid_, (const double*)s_csc_, (uint32_t)k,
-                                    (const uint32_t*)s_cidx_, nf, rids, rsc, ostride, stream_),
+      HIPCHK(launch_scatter_results((const uint64_t*)c_->s_cid_, (const double*)c_->s_csc_, (uint32_t)k,
+                                    (const uint32_t*)c_->s_cidx_, nf, rids, rsc, ostride, c_->stream),
              "k_scatter_results");
-      HIPCHK(hipStreamSynchronize(stream_), "second pass");
+      HIPCHK(hipStreamSynchronize(c_->stream), "second pass");
       continue;
     }
-    if ((rc = scratch(&s_fq_, &s_fq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
-    if ((rc = scratch(&s_fid_, &s_fid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
-    if ((rc = scratch(&s_fsc_, &s_fsc_n_, (size_t)nf * k * sizeof(double)))) return rc;
-    if ((rc = scratch(&s_fk_, &s_fk_n_, (size_t)nf * sizeof(int)))) return rc;
+    if ((rc = scratch(&c_->s_fq_, &c_->s_fq_n_, (size_t)nf * d_ * sizeof(double)))) return rc;
+    if ((rc = scratch(&c_->s_fid_, &c_->s_fid_n_, (size_t)nf * k * sizeof(uint64_t)))) return rc;
+    if ((rc = scratch(&c_->s_fsc_, &c_->s_fsc_n_, (size_t)nf * k * sizeof(double)))) return rc;
+    if ((rc = scratch(&c_->s_fk_, &c_->s_fk_n_, (size_t)nf * sizeof(int)))) return rc;
     std::vector<int> hk(nf, k);
-    HIPCHK(hipMemcpyAsync(s_fk_, hk.data(), nf * sizeof(int), hipMemcpyHostToDevice, stream_), "copy");
-    HIPCHK(launch_gather_rows(rQ, ldq, fail_list, nf, (uint32_t)d_, (double*)s_fq_, d_, stream_),
+    HIPCHK(hipMemcpyAsync(c_->s_fk_, hk.data(), nf * sizeof(int), hipMemcpyHostToDevice, c_->stream), "copy");
+    HIPCHK(launch_gather_rows(rQ, ldq, fail_list, nf, (uint32_t)d_, (double*)c_->s_fq_, d_, c_->stream),
            "k_gather_rows");
-    rc = run_exact((const double*)s_fq_, d_, nf, (uint32_t)d_, (const int*)s_fk_, metric, k,
-                   (uint64_t*)s_fid_, (double*)s_fsc_, (uint32_t)k);
+    rc = run_exact((const double*)c_->s_fq_, d_, nf, (uint32_t)d_, (const int*)c_->s_fk_, metric, k,
+                   (uint64_t*)c_->s_fid_, (double*)c_->s_fsc_, (uint32_t)k);
     if (rc) return rc;
-    HIPCHK(launch_scatter_results((const uint64_t*)s_fid_, (const double*)s_fsc_, (uint32_t)k,
-                                  fail_list, nf, rids, rsc, ostride, stream_),
+    HIPCHK(launch_scatter_results((const uint64_t*)c_->s_fid_, (const double*)c_->s_fsc_, (uint32_t)k,
+                                  fail_list, nf, rids, rsc, ostride, c_->stream),
            "k_scatter_results");
-    HIPCHK(hipStreamSynchronize(stream_), "fallback");
+    HIPCHK(hipStreamSynchronize(c_->stream), "fallback");
     st_.fallbacks += nf;
   }
   st_.last_max_rel_err = mx;
+  c_->lp.failed = lp.failed;
   return SVDB_OK;
 }
+
+int Engine::run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, int k,
+                            uint64_t* dids, double* dscores, uint32_t ostride, bool fp8) {
+  int rc = lowp_enqueue(dQ, ldq, R, nr, k, dids, dscores, ostride, fp8);
+  if (rc) {
+    c_->lp.on = false;
+    return rc;
+  }
+  return lowp_complete();
+}
+
+
 
 int Engine::use_lowp(int precision, size_t D, int* rc) {
   *rc = SVDB_OK;
@@ -1273,6 +1408,7 @@ static int check_search(size_t d, size_t D, int metric, int precision) {
 int Engine::search_device(const double* dQ, size_t nq, size_t ldq, size_t D, size_t k,
                           int metric, int precision, uint64_t* dids, double* dscores) {
   std::lock_guard<std::mutex> g(mu_);
+  c_ = &ctx_[0];
   if (!ok_) return SVDB_EDEVICE;
   if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
   if (d_ == 0) {
@@ -1286,9 +1422,9 @@ int Engine::search_device(const double* dQ, size_t nq, size_t ldq, size_t D, siz
     return SVDB_EINVAL;
   }
   const size_t nb = std::min(nq, std::max(kMaxBatch, kMaxBatchLowp));
-  if ((rc = scratch(&s_k_, &s_k_n_, nb * sizeof(int)))) return rc;
+  if ((rc = scratch(&c_->s_k_, &c_->s_k_n_, nb * sizeof(int)))) return rc;
   std::vector<int> hk(nb, (int)k);
-  HIPCHK(hipMemcpyAsync(s_k_, hk.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
+  HIPCHK(hipMemcpyAsync(c_->s_k_, hk.data(), nb * sizeof(int), hipMemcpyHostToDevice, c_->stream), "k upload");
   const int which = nphys_ > 0 ? use_lowp(precision, D, &rc) : 0;
   if (rc) return rc;
   const bool lowp = which != 0;
@@ -1297,12 +1433,65 @@ int Engine::search_device(const double* dQ, size_t nq, size_t ldq, size_t D, siz
     const uint32_t n = (uint32_t)std::min(step, nq - q0);
     rc = lowp ? run_lowp(dQ + q0 * ldq, ldq, n, metric, (int)k, dids + q0 * k, dscores + q0 * k,
                          (uint32_t)k, which == 2)
-              : run_exact(dQ + q0 * ldq, ldq, n, (uint32_t)D, (const int*)s_k_, metric, (int)k,
+              : run_exact(dQ + q0 * ldq, ldq, n, (uint32_t)D, (const int*)c_->s_k_, metric, (int)k,
                           dids + q0 * k, dscores + q0 * k, (uint32_t)k);
     if (rc) return rc;
   }
-  HIPCHK(hipStreamSynchronize(stream_), "search");
+  HIPCHK(hipStreamSynchronize(c_->stream), "search");
   prof_collect();
+  st_.searches++;
+  st_.queries += nq;
+  return SVDB_OK;
+}
+
+int Engine::search_device_noalloc(const double* dQ, size_t nq, size_t ldq, size_t k, int metric,
+                                  uint64_t* dids, double* dscores, void* scratch, size_t bytes) {
+  std::lock_guard<std::mutex> g(mu_);
+  c_ = &ctx_[0];
+  if (!ok_) return SVDB_EDEVICE;
+  if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
+  if (d_ == 0) {
+    set_error_static(SVDB_EEMPTY, "store has no rows");
+    return SVDB_EEMPTY;
+  }
+  if (metric < 0 || metric > 2 || k < 1 || k > SVDB_MAX_K || ldq < d_) {
+    set_error_static(SVDB_EINVAL, "bad argument: metric, k in [1, 128], or ldq below the "
+                                  "store's dimension");
+    return SVDB_EINVAL;
+  }
+  if (nq == 0) return SVDB_OK;
+  // scratch: k per query | query norms | partial keys | partial slots
+  char* base = static_cast<char*>(scratch);
+  const size_t kb = round_up(nq * sizeof(int), 256), qb = round_up(nq * sizeof(double), 256);
+  if (nq > kMaxBatch || kb + qb >= bytes) {
+    set_error_static(SVDB_EINVAL, "batch too large for the allocation-free search");
+    return SVDB_EINVAL;
+  }
+  const size_t entries = (bytes - kb - qb - 256) / (sizeof(uint64_t) + sizeof(uint32_t));
+  const uint32_t nslices = exact_slices((uint32_t)nq, (int)k, entries);
+  if (nphys_ > 0 && nslices == 0) {
+    set_error_static(SVDB_EINVAL, "queries x k too large for the allocation-free scratch");
+    return SVDB_EINVAL;
+  }
+  const size_t nlist = (size_t)nq * nslices * kWavesPerBlock * k;
+  int* dk = reinterpret_cast<int*>(base);
+  double* qn = reinterpret_cast<double*>(base + kb);
+  uint64_t* pk = reinterpret_cast<uint64_t*>(base + kb + qb);
+  uint32_t* ps = reinterpret_cast<uint32_t*>(base + kb + qb + round_up(nlist * 8, 256));
+  if (launch_fill_i32(dk, (uint32_t)nq, (int)k, c_->stream) != hipSuccess) {
+    set_error_static(SVDB_EDEVICE, "k fill failed");
+    return SVDB_EDEVICE;
+  }
+  const bool prof = prof_;
+  prof_ = false;  // the profiling slots create events
+  int rc = run_exact_with(dQ, ldq, (uint32_t)nq, (uint32_t)d_, dk, metric, (int)k, dids, dscores,
+                          (uint32_t)k, nslices, pk, ps, qn);
+  prof_ = prof;
+  if (rc) return rc;
+  if (hipStreamSynchronize(c_->stream) != hipSuccess) {
+    set_error_static(SVDB_EDEVICE, "search failed on the device");
+    return SVDB_EDEVICE;
+  }
   st_.searches++;
   st_.queries += nq;
   return SVDB_OK;
@@ -1312,6 +1501,66 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
                         const int* metric, int precision, size_t ostride, size_t* ids,
                         double* scores) {
   std::lock_guard<std::mutex> g(mu_);
+  SearchCtx& c = ctx_[0];
+  int rc = search_begin_locked(c, Q, nq, ldq, D, k, metric, precision, ostride, ids, scores);
+  if (rc == SVDB_OK) rc = search_complete(c);
+  c.state = 0;
+  c_ = &ctx_[0];
+  return rc;
+}
+
+int Engine::search_begin(int slot, const double* Q, size_t nq, size_t ldq, size_t D,
+                         const size_t* k, const int* metric, int precision, size_t ostride,
+                         size_t* ids, double* scores) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (slot < 1 || slot >= kCtx || ctx_[slot].state != 0) {
+    set_error(SVDB_EINVAL, "search slot busy or out of range");
+    return SVDB_EINVAL;
+  }
+  SearchCtx& c = ctx_[slot];
+  int rc = search_begin_locked(c, Q, nq, ldq, D, k, metric, precision, ostride, ids, scores);
+  if (rc != SVDB_OK) c.state = 0;
+  c_ = &ctx_[0];
+  return rc;
+}
+
+int Engine::search_end(int slot) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (slot < 1 || slot >= kCtx || ctx_[slot].state == 0) {
+    set_error(SVDB_EINVAL, "no search in flight in this slot");
+    return SVDB_EINVAL;
+  }
+  SearchCtx& c = ctx_[slot];
+  const int rc = c.state == 1 ? search_complete(c) : c.rc;
+  c.state = 0;
+  c_ = &ctx_[0];
+  return rc;
+}
+
+// (under mu_) completes the searches begun and not yet ended, so that a
+// store mutation cannot change what they see; their results wait in the
+// callers' arrays for search_end.
+int Engine::complete_pending() {
+  for (int i = 1; i < kCtx; i++)
+    if (ctx_[i].state == 1) {
+      ctx_[i].rc = search_complete(ctx_[i]);
+      ctx_[i].state = 2;
+    }
+  c_ = &ctx_[0];
+  return SVDB_OK;
+}
+
+// Starts a host-query search in context c.  The BF16 form over rows <= 256
+// wide (the serving case) is only enqueued: query upload, the first pass,
+// shortlists, reranks and the result download run on c's stream, and
+// search_complete finishes.  Every other form runs to completion here.
+// c.state: 1 enqueued, 2 complete (c.rc).
+int Engine::search_begin_locked(SearchCtx& c, const double* Q, size_t nq, size_t ldq, size_t D,
+                                const size_t* k, const int* metric, int precision,
+                                size_t ostride, size_t* ids, double* scores) {
+  c_ = &c;
+  c.state = 2;
+  c.rc = SVDB_OK;
   if (!ok_) return SVDB_EDEVICE;
   if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
   if (nq == 0) return SVDB_OK;
@@ -1345,7 +1594,7 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   const bool lowp = which != 0;
   const size_t step = lowp ? kMaxBatchLowp : kMaxBatch;
 
-  std::vector<size_t> sel;
+  std::vector<size_t>& sel = c.sel;
   // Host side of one chunk (query rows, k values, ids, scores): pinned
   // staging when it fits under its cap, else pageable vectors.
   std::vector<double> hq, hsc;
@@ -1358,69 +1607,75 @@ int Engine::search_host(const double* Q, size_t nq, size_t ldq, size_t D, const 
   auto host_bufs = [&](size_t n, size_t kk) {
     const size_t qb = round_up(n * D * sizeof(double), 64), kb = round_up(n * sizeof(int), 64),
                  ob = round_up(n * kk * sizeof(uint64_t), 64);
-    if (char* st = static_cast<char*>(host_stage(qb + kb + 2 * ob))) {
+    if (char* st = static_cast<char*>(host_stage(&c.h_stage_, &c.h_stage_n_, qb + kb + 2 * ob))) {
       pq = reinterpret_cast<double*>(st);
       pk = reinterpret_cast<int*>(st + qb);
       pid = reinterpret_cast<uint64_t*>(st + qb + kb);
       psc = reinterpret_cast<double*>(st + qb + kb + ob);
-    } else {
-      hq.resize(n * D);
-      hk.resize(n);
-      hid.resize(n * kk);
-      hsc.resize(n * kk);
-      pq = hq.data();
-      pk = hk.data();
-      pid = hid.data();
-      psc = hsc.data();
+      return true;
     }
+    hq.resize(n * D);
+    hk.resize(n);
+    hid.resize(n * kk);
+    hsc.resize(n * kk);
+    pq = hq.data();
+    pk = hk.data();
+    pid = hid.data();
+    psc = hsc.data();
+    return false;
   };
-  // Several metrics in one BF16 batch (rows <= 256 wide): one first pass
-  // for all of them, queries grouped by metric.
+  // The BF16 first pass over rows <= 256 wide: one pass for every metric of
+  // the batch, queries grouped by metric into ranges, all at the batch's
+  // largest k (a wavefront of k > 32 queries in a block sets that block's
+  // pace anyway, and extra ranges cost more in latency-bound shortlist and
+  // rerank launches than their narrower lists save).  Enqueued only.
   if (which == 1 && sh_.dp <= 256 && nq <= step) {
-    {
-      // one range per metric, all at the batch's largest k: a wavefront of
-      // k > 32 queries in a block sets that block's pace anyway, and extra
-      // ranges cost more in latency-bound shortlist and rerank launches than
-      // their narrower lists save
-      LowpRange R[3];
-      int nr = 0;
-      sel.clear();
-      for (int m = 0; m < 3; m++) {
-        const size_t first = sel.size();
-        for (size_t i = 0; i < nq; i++)
-          if (metric[i] == m) sel.push_back(i);
-        if (sel.size() > first)
-          R[nr++] = LowpRange{m, (uint32_t)first, (uint32_t)(sel.size() - first),
-                              (int)kmax_all, 0, 0, 0, 0, 0};
-      }
-      if (nr < 2) goto per_metric;  // one metric: the plain path below
-      const size_t n = sel.size();
-      if ((rc = scratch(&s_q_, &s_q_n_, n * D * sizeof(double)))) return rc;
-      if ((rc = scratch(&s_ids_, &s_ids_n_, n * kmax_all * sizeof(uint64_t)))) return rc;
-      if ((rc = scratch(&s_sc_, &s_sc_n_, n * kmax_all * sizeof(double)))) return rc;
-      host_bufs(n, kmax_all);
-      for (size_t t = 0; t < n; t++) std::memcpy(pq + t * D, Q + sel[t] * ldq, D * sizeof(double));
-      HIPCHK(hipMemcpyAsync(s_q_, pq, n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
-      rc = run_lowp_ranges((const double*)s_q_, D, R, nr, (int)kmax_all, (uint64_t*)s_ids_,
-                           (double*)s_sc_, (uint32_t)kmax_all, false);
-      if (rc) return rc;
-      HIPCHK(hipMemcpyAsync(pid, s_ids_, n * kmax_all * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
-      HIPCHK(hipMemcpyAsync(psc, s_sc_, n * kmax_all * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
-      HIPCHK(hipStreamSynchronize(stream_), "search");
-      prof_collect();
-      for (size_t t = 0; t < n; t++) {
-        const size_t i = sel[t];
-        for (size_t j = 0; j < k[i]; j++) {
-          ids[i * ostride + j] = (size_t)pid[t * kmax_all + j];
-          scores[i * ostride + j] = psc[t * kmax_all + j];
-        }
-      }
-      st_.searches++;
-      st_.queries += nq;
-      return SVDB_OK;
+    LowpRange R[3];
+    int nr = 0;
+    sel.clear();
+    for (int m = 0; m < 3; m++) {
+      const size_t first = sel.size();
+      for (size_t i = 0; i < nq; i++)
+        if (metric[i] == m) sel.push_back(i);
+      if (sel.size() > first)
+        R[nr++] = LowpRange{m, (uint32_t)first, (uint32_t)(sel.size() - first), (int)kmax_all,
+                            0, 0, 0, 0, 0};
     }
+    const size_t n = sel.size();
+    if ((rc = scratch(&c.s_q_, &c.s_q_n_, n * D * sizeof(double)))) return rc;
+    if ((rc = scratch(&c.s_ids_, &c.s_ids_n_, n * kmax_all * sizeof(uint64_t)))) return rc;
+    if ((rc = scratch(&c.s_sc_, &c.s_sc_n_, n * kmax_all * sizeof(double)))) return rc;
+    if (!host_bufs(n, kmax_all)) {
+      set_error(SVDB_ENOMEM, "pinned staging unavailable");
+      return SVDB_ENOMEM;
+    }
+    c.kq.assign(k, k + nq);
+    c.nq = nq;
+    c.kmax = kmax_all;
+    c.ostride = ostride;
+    c.ids = ids;
+    c.scores = scores;
+    c.pid = pid;
+    c.psc = psc;
+    for (size_t t = 0; t < n; t++) std::memcpy(pq + t * D, Q + sel[t] * ldq, D * sizeof(double));
+    HIPCHK(hipMemcpyAsync(c.s_q_, pq, n * D * sizeof(double), hipMemcpyHostToDevice, c.stream),
+           "q upload");
+    rc = lowp_enqueue((const double*)c.s_q_, D, R, nr, (int)kmax_all, (uint64_t*)c.s_ids_,
+                      (double*)c.s_sc_, (uint32_t)kmax_all, false);
+    if (rc) {
+      c.lp.on = false;
+      (void)hipStreamSynchronize(c.stream);
+      return rc;
+    }
+    // the results of every query the first pass certifies (in practice all
+    // of them): downloaded now, before anyone waits
+    HIPCHK(hipMemcpyAsync(pid, c.s_ids_, n * kmax_all * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          c.stream), "ids download");
+    HIPCHK(hipMemcpyAsync(psc, c.s_sc_, n * kmax_all * sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream), "scores download");
+    c.state = 1;
+    return SVDB_OK;
   }
-per_metric:
   for (int m = 0; m < 3; m++) {
     sel.clear();
     for (size_t i = 0; i < nq; i++)
@@ -1429,26 +1684,31 @@ per_metric:
       const size_t n = std::min(step, sel.size() - b0);
       size_t kmax = 0;
       for (size_t t = 0; t < n; t++) kmax = std::max(kmax, k[sel[b0 + t]]);
-      if ((rc = scratch(&s_q_, &s_q_n_, n * D * sizeof(double)))) return rc;
-      if ((rc = scratch(&s_k_, &s_k_n_, n * sizeof(int)))) return rc;
-      if ((rc = scratch(&s_ids_, &s_ids_n_, n * kmax * sizeof(uint64_t)))) return rc;
-      if ((rc = scratch(&s_sc_, &s_sc_n_, n * kmax * sizeof(double)))) return rc;
+      if ((rc = scratch(&c.s_q_, &c.s_q_n_, n * D * sizeof(double)))) return rc;
+      if ((rc = scratch(&c.s_k_, &c.s_k_n_, n * sizeof(int)))) return rc;
+      if ((rc = scratch(&c.s_ids_, &c.s_ids_n_, n * kmax * sizeof(uint64_t)))) return rc;
+      if ((rc = scratch(&c.s_sc_, &c.s_sc_n_, n * kmax * sizeof(double)))) return rc;
       host_bufs(n, kmax);
       for (size_t t = 0; t < n; t++) {
         const size_t i = sel[b0 + t];
         std::memcpy(pq + t * D, Q + i * ldq, D * sizeof(double));
         pk[t] = (int)k[i];
       }
-      HIPCHK(hipMemcpyAsync(s_q_, pq, n * D * sizeof(double), hipMemcpyHostToDevice, stream_), "q upload");
-      HIPCHK(hipMemcpyAsync(s_k_, pk, n * sizeof(int), hipMemcpyHostToDevice, stream_), "k upload");
-      rc = lowp ? run_lowp((const double*)s_q_, D, (uint32_t)n, m, (int)kmax, (uint64_t*)s_ids_,
-                           (double*)s_sc_, (uint32_t)kmax, which == 2)
-                : run_exact((const double*)s_q_, D, (uint32_t)n, (uint32_t)D, (const int*)s_k_, m,
-                            (int)kmax, (uint64_t*)s_ids_, (double*)s_sc_, (uint32_t)kmax);
+      HIPCHK(hipMemcpyAsync(c.s_q_, pq, n * D * sizeof(double), hipMemcpyHostToDevice, c.stream),
+             "q upload");
+      HIPCHK(hipMemcpyAsync(c.s_k_, pk, n * sizeof(int), hipMemcpyHostToDevice, c.stream),
+             "k upload");
+      rc = lowp ? run_lowp((const double*)c.s_q_, D, (uint32_t)n, m, (int)kmax,
+                           (uint64_t*)c.s_ids_, (double*)c.s_sc_, (uint32_t)kmax, which == 2)
+                : run_exact((const double*)c.s_q_, D, (uint32_t)n, (uint32_t)D,
+                            (const int*)c.s_k_, m, (int)kmax, (uint64_t*)c.s_ids_,
+                            (double*)c.s_sc_, (uint32_t)kmax);
       if (rc) return rc;
-      HIPCHK(hipMemcpyAsync(pid, s_ids_, n * kmax * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_), "ids download");
-      HIPCHK(hipMemcpyAsync(psc, s_sc_, n * kmax * sizeof(double), hipMemcpyDeviceToHost, stream_), "scores download");
-      HIPCHK(hipStreamSynchronize(stream_), "search");
+      HIPCHK(hipMemcpyAsync(pid, c.s_ids_, n * kmax * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            c.stream), "ids download");
+      HIPCHK(hipMemcpyAsync(psc, c.s_sc_, n * kmax * sizeof(double), hipMemcpyDeviceToHost,
+                            c.stream), "scores download");
+      HIPCHK(hipStreamSynchronize(c.stream), "search");
       prof_collect();
       for (size_t t = 0; t < n; t++) {
         const size_t i = sel[b0 + t];
@@ -1464,10 +1724,42 @@ per_metric:
   return SVDB_OK;
 }
 
+// Finishes a search search_begin_locked enqueued in c: waits for it, runs
+// the retries of the queries the first pass could not certify, and writes
+// the caller's ids and scores.
+int Engine::search_complete(SearchCtx& c) {
+  if (c.state != 1) return c.rc;
+  c_ = &c;
+  c.state = 2;
+  int rc = lowp_complete();
+  if (rc) return c.rc = rc;
+  const size_t n = c.sel.size(), kk = c.kmax;
+  if (c.lp.failed) {
+    // retries rewrote some queries' results on the device
+    HIPCHK(hipMemcpyAsync(c.pid, c.s_ids_, n * kk * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          c.stream), "ids download");
+    HIPCHK(hipMemcpyAsync(c.psc, c.s_sc_, n * kk * sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream), "scores download");
+  }
+  HIPCHK(hipStreamSynchronize(c.stream), "search");
+  prof_collect();
+  for (size_t t = 0; t < n; t++) {
+    const size_t i = c.sel[t];
+    for (size_t j = 0; j < c.kq[i]; j++) {
+      c.ids[i * c.ostride + j] = (size_t)c.pid[t * kk + j];
+      c.scores[i * c.ostride + j] = c.psc[t * kk + j];
+    }
+  }
+  st_.searches++;
+  st_.queries += c.nq;
+  return c.rc = SVDB_OK;
+}
+
 // ------------------------------------------------------------------ pairs
 
 int Engine::pairs_f64(const size_t* a, const size_t* b, size_t np, int metric, double* out) {
   std::lock_guard<std::mutex> g(mu_);
+  c_ = &ctx_[0];
   if (!ok_) return SVDB_EDEVICE;
   if (hipSetDevice(device_) != hipSuccess) return SVDB_EDEVICE;
   if (metric < 0 || metric > 2) {
@@ -1486,14 +1778,14 @@ int Engine::pairs_f64(const size_t* a, const size_t* b, size_t np, int metric, d
     hs[np + p] = select_slot(b[p]);
   }
   int rc;
-  if ((rc = scratch(&s_pair_, &s_pair_n_, 2 * np * sizeof(uint32_t)))) return rc;
-  if ((rc = scratch(&s_pout_, &s_pout_n_, np * sizeof(double)))) return rc;
-  HIPCHK(hipMemcpyAsync(s_pair_, hs.data(), 2 * np * sizeof(uint32_t), hipMemcpyHostToDevice, stream_), "pairs upload");
-  const uint32_t* sa = (const uint32_t*)s_pair_;
+  if ((rc = scratch(&c_->s_pair_, &c_->s_pair_n_, 2 * np * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&c_->s_pout_, &c_->s_pout_n_, np * sizeof(double)))) return rc;
+  HIPCHK(hipMemcpyAsync(c_->s_pair_, hs.data(), 2 * np * sizeof(uint32_t), hipMemcpyHostToDevice, c_->stream), "pairs upload");
+  const uint32_t* sa = (const uint32_t*)c_->s_pair_;
   HIPCHK(launch_pair_f64(X_, ld_, sa, sa + np, (uint32_t)np, (uint32_t)d_, metric,
-                         (double*)s_pout_, stream_), "k_pair_f64");
-  HIPCHK(hipMemcpyAsync(out, s_pout_, np * sizeof(double), hipMemcpyDeviceToHost, stream_), "pairs download");
-  HIPCHK(hipStreamSynchronize(stream_), "pairs");
+                         (double*)c_->s_pout_, c_->stream), "k_pair_f64");
+  HIPCHK(hipMemcpyAsync(out, c_->s_pout_, np * sizeof(double), hipMemcpyDeviceToHost, c_->stream), "pairs download");
+  HIPCHK(hipStreamSynchronize(c_->stream), "pairs");
   return SVDB_OK;
 }
 

@@ -32,6 +32,7 @@
 namespace svdb {
 
 void set_error(int code, const std::string& msg);
+void set_error_static(int code, const char* msg);  // allocates nothing
 const char* last_error();
 
 // Host-allocation fault injection for tests (svdb_debug_fail_host_alloc):
@@ -92,9 +93,23 @@ class Engine {
   // Host queries; per-query k and metric; ids/scores row stride ostride.
   int search_host(const double* Q, size_t nq, size_t ldq, size_t D, const size_t* k,
                   const int* metric, int precision, size_t ostride, size_t* ids, double* scores);
+  // Split form of search_host for the serving loop: begin enqueues the
+  // search on context `slot` (1 or 2) and returns; end completes it (waits,
+  // runs any retries) and writes ids / scores, which must stay valid from
+  // begin to end.  Two searches may be in flight at once; a store mutation
+  // completes them first.
+  int search_begin(int slot, const double* Q, size_t nq, size_t ldq, size_t D, const size_t* k,
+                   const int* metric, int precision, size_t ostride, size_t* ids,
+                   double* scores);
+  int search_end(int slot);
   // Device queries and outputs; one k and metric.
   int search_device(const double* dQ, size_t nq, size_t ldq, size_t D, size_t k, int metric,
                     int precision, uint64_t* dids, double* dscores);
+  // The FP64 exact scan of search_device with every intermediate in the
+  // caller's device scratch (`bytes` at `scratch`, 256-byte aligned):
+  // allocates nothing, host or device.
+  int search_device_noalloc(const double* dQ, size_t nq, size_t ldq, size_t k, int metric,
+                            uint64_t* dids, double* dscores, void* scratch, size_t bytes);
 
   // ---- pair metrics over stored rows (logical indices) ----
   int pairs_f64(const size_t* a, const size_t* b, size_t np, int metric, double* out);
@@ -121,9 +136,14 @@ class Engine {
   int dev_alloc(void** p, size_t bytes);
   void dev_free(void* p, size_t bytes);
   int scratch(void** p, size_t* have, size_t need);
-  void* host_stage(size_t bytes);  // pinned staging, grow-only; nullptr past its cap
+  // pinned staging, grow-only; nullptr past its cap
+  void* host_stage(void** buf, size_t* have, size_t bytes);
   int run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk, int metric,
                 int kmax, uint64_t* dids, double* dscores, uint32_t ostride);
+  int run_exact_with(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk,
+                     int metric, int kmax, uint64_t* dids, double* dscores, uint32_t ostride,
+                     uint32_t nslices, uint64_t* pk, uint32_t* ps, double* qn);
+  uint32_t exact_slices(uint32_t nq, int kmax, size_t max_entries) const;
   struct LowpRange {
     int metric;
     uint32_t qoff, n;  // query rows qoff .. qoff+n-1
@@ -134,6 +154,7 @@ class Engine {
     uint32_t ns, rps, NL;
     size_t coff;
   };
+  static uint32_t lowp_width(int kk, bool fp8, size_t shortlist);
   int run_lowp(const double* dQ, size_t ldq, uint32_t nq, int metric, int k, uint64_t* dids,
                double* dscores, uint32_t ostride, bool fp8);
   int run_lowp_ranges(const double* dQ, size_t ldq, LowpRange* R, int nr, int k, uint64_t* dids,
@@ -181,50 +202,98 @@ class Engine {
   size_t budget_ = SIZE_MAX;
   size_t shortlist_ = 64;
 
-  // scratch (grow-only)
-  void* s_q_ = nullptr;      size_t s_q_n_ = 0;
-  void* s_qn_ = nullptr;     size_t s_qn_n_ = 0;
-  void* s_k_ = nullptr;      size_t s_k_n_ = 0;
-  void* s_pk_ = nullptr;     size_t s_pk_n_ = 0;
-  void* s_ps_ = nullptr;     size_t s_ps_n_ = 0;
-  void* s_ids_ = nullptr;    size_t s_ids_n_ = 0;
-  void* s_sc_ = nullptr;     size_t s_sc_n_ = 0;
-  void* s_pair_ = nullptr;   size_t s_pair_n_ = 0;
-  void* s_pout_ = nullptr;   size_t s_pout_n_ = 0;
-  void* s_qb_ = nullptr;     size_t s_qb_n_ = 0;
-  void* s_qp_ = nullptr;     size_t s_qp_n_ = 0;
-  void* s_ck_ = nullptr;     size_t s_ck_n_ = 0;
-  void* s_cs_ = nullptr;     size_t s_cs_n_ = 0;
-  void* s_sk_ = nullptr;     size_t s_sk_n_ = 0;
-  void* s_ss_ = nullptr;     size_t s_ss_n_ = 0;
-  void* s_beta_ = nullptr;   size_t s_beta_n_ = 0;
-  void* s_fail_ = nullptr;   size_t s_fail_n_ = 0;
-  void* s_err_ = nullptr;    size_t s_err_n_ = 0;
-  void* s_fq_ = nullptr;     size_t s_fq_n_ = 0;
-  void* s_fid_ = nullptr;    size_t s_fid_n_ = 0;
-  void* s_fsc_ = nullptr;    size_t s_fsc_n_ = 0;
-  void* s_fk_ = nullptr;     size_t s_fk_n_ = 0;
-  void* s_sk2_ = nullptr;    size_t s_sk2_n_ = 0;   // widening retry: shortlist keys
-  void* s_ss2_ = nullptr;    size_t s_ss2_n_ = 0;   //   slots
-  void* s_beta2_ = nullptr;  size_t s_beta2_n_ = 0; //   bounds
-  void* s_fail2_ = nullptr;  size_t s_fail2_n_ = 0; //   queries still uncertified
-  void* s_cidx_ = nullptr;   size_t s_cidx_n_ = 0;  // FP8 -> BF16 second pass: query indices
-  void* s_cq_ = nullptr;     size_t s_cq_n_ = 0;    //   gathered queries
-  void* s_cid_ = nullptr;    size_t s_cid_n_ = 0;   //   their ids
-  void* s_csc_ = nullptr;    size_t s_csc_n_ = 0;   //   and scores
-  uint32_t* h_fail_ = nullptr;  // pinned: fail count
-  void* h_io_ = nullptr;        // pinned: per-range fail counts + first-pass errors
-  size_t h_io_n_ = 0;
-  void* h_stage_ = nullptr;     // pinned: row upload and search staging
-  size_t h_stage_n_ = 0;
-
   struct ProfSlot {
     int kind;
     hipEvent_t a, b;
   };
+
+  // Everything one search call owns while it runs: its stream, its
+  // grow-only device scratch and pinned staging, its profiling slots, and —
+  // for a search split into begin / end (search_begin) — what end needs.
+  // Context 0 serves the synchronous entry points; the serving loop keeps
+  // up to two searches in flight on contexts 1 and 2, whose kernels overlap
+  // on the device.  c_ is the context the current call works in (under mu_).
+  struct SearchCtx {
+    hipStream_t stream = nullptr;
+    void* s_q_ = nullptr;      size_t s_q_n_ = 0;
+    void* s_qn_ = nullptr;     size_t s_qn_n_ = 0;
+    void* s_k_ = nullptr;      size_t s_k_n_ = 0;
+    void* s_pk_ = nullptr;     size_t s_pk_n_ = 0;
+    void* s_ps_ = nullptr;     size_t s_ps_n_ = 0;
+    void* s_ids_ = nullptr;    size_t s_ids_n_ = 0;
+    void* s_sc_ = nullptr;     size_t s_sc_n_ = 0;
+    void* s_pair_ = nullptr;   size_t s_pair_n_ = 0;
+    void* s_pout_ = nullptr;   size_t s_pout_n_ = 0;
+    void* s_qb_ = nullptr;     size_t s_qb_n_ = 0;
+    void* s_qp_ = nullptr;     size_t s_qp_n_ = 0;
+    void* s_ck_ = nullptr;     size_t s_ck_n_ = 0;
+    void* s_cs_ = nullptr;     size_t s_cs_n_ = 0;
+    void* s_sk_ = nullptr;     size_t s_sk_n_ = 0;
+    void* s_ss_ = nullptr;     size_t s_ss_n_ = 0;
+    void* s_beta_ = nullptr;   size_t s_beta_n_ = 0;
+    void* s_fail_ = nullptr;   size_t s_fail_n_ = 0;
+    void* s_err_ = nullptr;    size_t s_err_n_ = 0;
+    void* s_fq_ = nullptr;     size_t s_fq_n_ = 0;
+    void* s_fid_ = nullptr;    size_t s_fid_n_ = 0;
+    void* s_fsc_ = nullptr;    size_t s_fsc_n_ = 0;
+    void* s_fk_ = nullptr;     size_t s_fk_n_ = 0;
+    void* s_sk2_ = nullptr;    size_t s_sk2_n_ = 0;   // widening retry: shortlist keys
+    void* s_ss2_ = nullptr;    size_t s_ss2_n_ = 0;   //   slots
+    void* s_beta2_ = nullptr;  size_t s_beta2_n_ = 0; //   bounds
+    void* s_fail2_ = nullptr;  size_t s_fail2_n_ = 0; //   queries still uncertified
+    void* s_cidx_ = nullptr;   size_t s_cidx_n_ = 0;  // FP8 -> BF16 second pass: query indices
+    void* s_cq_ = nullptr;     size_t s_cq_n_ = 0;    //   gathered queries
+    void* s_cid_ = nullptr;    size_t s_cid_n_ = 0;   //   their ids
+    void* s_csc_ = nullptr;    size_t s_csc_n_ = 0;   //   and scores
+    uint32_t* h_fail_ = nullptr;  // pinned: fail count
+    void* h_io_ = nullptr;        // pinned: per-range fail counts + first-pass errors
+    size_t h_io_n_ = 0;
+    void* h_stage_ = nullptr;     // pinned: search staging
+    size_t h_stage_n_ = 0;
+    std::vector<ProfSlot> prof_pending_;
+    // a low-precision pass enqueued and not yet completed (lowp_enqueue)
+    struct LowpPending {
+      bool on = false;
+      LowpRange R[3];
+      int nr = 0, k = 0;
+      const double* dQ = nullptr;
+      size_t ldq = 0;
+      uint64_t* dids = nullptr;
+      double* dscores = nullptr;
+      uint32_t ostride = 0;
+      bool fp8 = false;
+      uint32_t nq = 0;  // slots
+      size_t shortlist = 0;
+      bool failed = false;  // some query went through a retry (set by lowp_complete)
+    } lp;
+    // an asynchronous search (search_begin): 0 idle, 1 enqueued, 2 completed
+    int state = 0;
+    int rc = 0;
+    size_t nq = 0, kmax = 0, ostride = 0;
+    std::vector<size_t> sel;   // slot order -> query index
+    std::vector<size_t> kq;    // per query k
+    size_t* ids = nullptr;     // the caller's outputs
+    double* scores = nullptr;
+    uint64_t* pid = nullptr;   // pinned results (in h_stage_)
+    double* psc = nullptr;
+  };
+  static constexpr int kCtx = 3;
+  SearchCtx ctx_[kCtx];
+  SearchCtx* c_ = &ctx_[0];
+  void* h_stage_ = nullptr;  // pinned: row upload staging
+  size_t h_stage_n_ = 0;
+  int ctx_stream(SearchCtx& c);
+  int lowp_enqueue(const double* dQ, size_t ldq, LowpRange* R, int nr, int k, uint64_t* dids,
+                   double* dscores, uint32_t ostride, bool fp8);
+  int lowp_complete();
+  int search_begin_locked(SearchCtx& c, const double* Q, size_t nq, size_t ldq, size_t D,
+                          const size_t* k, const int* metric, int precision, size_t ostride,
+                          size_t* ids, double* scores);
+  int search_complete(SearchCtx& c);
+  int complete_pending();  // completes every begun search (before a store mutation)
+
   bool prof_ = false;
   bool prof_mute_ = false;  // inside the BF16 second pass: timed as a whole
-  std::vector<ProfSlot> prof_pending_;
   std::vector<hipEvent_t> ev_pool_;
   KernelTime kt_[SVDB_K_COUNT];
   EngineStats st_;

@@ -55,6 +55,8 @@ struct MergeArgs {
 };
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// p[0..n) = v
+hipError_t launch_fill_i32(int* p, uint32_t n, int v, hipStream_t s);
 hipError_t launch_row_norms(const double* X, size_t ldx, const uint32_t* slots, uint32_t first,
                             uint32_t n, uint32_t d, double* nx2, double* xn, hipStream_t s);
 hipError_t launch_query_norms(const double* Q, size_t ldq, uint32_t nq, uint32_t D, double* qn,

@@ -198,6 +198,17 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   return hipGetLastError();
 }
 
+__global__ void k_fill_i32(int* __restrict__ p, uint32_t n, int v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_i32(int* p, uint32_t n, int v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------- FP64 scan
 //
 // Grid (query groups of kScanTQ, row slices); 4 wavefronts per block share

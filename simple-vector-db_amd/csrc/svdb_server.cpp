@@ -262,15 +262,48 @@ class Server {
     return t.nq + r.nq <= cfg_.max_batch && t.bytes + r.est_bytes <= cfg_.budget_bytes;
   }
 
+  // A search tick on the device: begun in an engine slot (1 or 2), its
+  // results land in ids / sc when it is ended.
+  struct Flight {
+    std::unique_ptr<Tick> t;
+    int slot = 0;
+    int64_t t0 = 0;
+    uint64_t tick = 0;
+    int rc = SVDB_OK;
+    std::vector<size_t> ids;
+    std::vector<double> sc;
+  };
+
+  // Executor: keeps up to two search ticks in flight, so one tick's first
+  // pass runs on the device while the previous one's shortlists, reranks
+  // and result copies finish and its requests are answered.  A mutation
+  // tick waits until no search is in flight (those searches were submitted
+  // before it), then applies its mutations in order.
   void exec_loop() {
+    std::deque<std::unique_ptr<Flight>> fl;
     for (;;) {
       std::unique_ptr<Tick> t;
+      bool finish_oldest = false;
       {
         std::unique_lock<std::mutex> g(mu_);
-        cv_tick_.wait(g, [&] { return stop_ || staged_; });
-        if (!staged_ && stop_) return;
-        t = std::move(staged_);
-        t->mirror_seq = mirror_issued_;
+        if (fl.empty()) cv_tick_.wait(g, [&] { return stop_ || staged_; });
+        const bool can_take =
+            staged_ && (staged_->mutations.empty() ? fl.size() < 2 : fl.empty());
+        if (can_take) {
+          t = std::move(staged_);
+          t->mirror_seq = mirror_issued_;
+        } else if (!fl.empty()) {
+          finish_oldest = true;
+        } else if (stop_) {
+          return;
+        } else {
+          continue;
+        }
+      }
+      if (finish_oldest) {
+        end_flight(*fl.front());
+        fl.pop_front();
+        continue;
       }
       cv_q_.notify_all();  // the stager may form the next tick now
       const int64_t t0 = now_us();
@@ -306,51 +339,80 @@ class Server {
         cv_mirror_.notify_all();
         continue;
       }
+      // a search tick: begin it in the free slot
+      std::unique_ptr<Flight> f(new (std::nothrow) Flight);
+      if (!f) {
+        fail_tick(*t, SVDB_ENOMEM, t0, tick);
+        continue;
+      }
+      f->slot = (!fl.empty() && fl.front()->slot == 1) ? 2 : 1;
+      f->t0 = t0;
+      f->tick = tick;
       const size_t d = eng_->dim();
-      std::vector<size_t> ids;
-      std::vector<double> sc;
-      int rc = SVDB_OK;
       try {  // a host allocation failure fails the tick's searches
         fault_point();
-        ids.resize(t->nq * t->kmax);
-        sc.resize(t->nq * t->kmax);
-        if (t->nq)
-          rc = eng_->search_host(t->q.data(), t->nq, d, d, t->k.data(), t->metric.data(),
-                                 cfg_.precision, t->kmax, ids.data(), sc.data());
+        f->ids.resize(t->nq * t->kmax);
+        f->sc.resize(t->nq * t->kmax);
+        f->rc = eng_->search_begin(f->slot, t->q.data(), t->nq, d, d, t->k.data(),
+                                   t->metric.data(), cfg_.precision, t->kmax, f->ids.data(),
+                                   f->sc.data());
       } catch (const std::bad_alloc&) {
-        rc = SVDB_ENOMEM;
+        f->rc = SVDB_ENOMEM;
       } catch (...) {
-        rc = SVDB_EINVAL;
+        f->rc = SVDB_EINVAL;
       }
-      size_t o = 0;
-      for (auto& r : t->reqs) {
-        r->rc = rc;
-        if (rc == SVDB_OK) {
-          try {
-            fault_point();
-            r->ids.resize(r->nq * r->k);
-            r->scores.resize(r->nq * r->k);
-          } catch (const std::bad_alloc&) {
-            r->rc = SVDB_ENOMEM;
-            o += r->nq;
-            continue;
-          }
-          for (size_t i = 0; i < r->nq; i++)
-            for (size_t j = 0; j < r->k; j++) {
-              r->ids[i * r->k + j] = ids[(o + i) * t->kmax + j];
-              r->scores[i * r->k + j] = sc[(o + i) * t->kmax + j];
-            }
-        }
-        o += r->nq;
+      f->t = std::move(t);
+      if (f->rc != SVDB_OK) {  // nothing in flight for this tick
+        fail_tick(*f->t, f->rc, t0, tick);
+        continue;
       }
-      // the ids name rows of the store as the mutations before this tick
-      // left it: the mirror (vector_db_read) must show that state first
-      {
-        std::unique_lock<std::mutex> g(mu_);
-        cv_applied_.wait(g, [&] { return mirror_applied_ >= t->mirror_seq; });
+      try {
+        fl.push_back(std::move(f));
+      } catch (...) {
+        // (the deque could not grow: answer this tick now)
+        end_flight(*f);
       }
-      finish(t->reqs, t0, tick, t->nq);
     }
+  }
+
+  // Ends a flight: waits for its search, hands each request its rows of the
+  // results, and completes them once the mirror shows the store they saw.
+  void end_flight(Flight& f) {
+    int rc = eng_->search_end(f.slot);
+    Tick& t = *f.t;
+    size_t o = 0;
+    for (auto& r : t.reqs) {
+      r->rc = rc;
+      if (rc == SVDB_OK) {
+        try {
+          fault_point();
+          r->ids.resize(r->nq * r->k);
+          r->scores.resize(r->nq * r->k);
+        } catch (const std::bad_alloc&) {
+          r->rc = SVDB_ENOMEM;
+          o += r->nq;
+          continue;
+        }
+        for (size_t i = 0; i < r->nq; i++)
+          for (size_t j = 0; j < r->k; j++) {
+            r->ids[i * r->k + j] = f.ids[(o + i) * t.kmax + j];
+            r->scores[i * r->k + j] = f.sc[(o + i) * t.kmax + j];
+          }
+      }
+      o += r->nq;
+    }
+    // the ids name rows of the store as the mutations before this tick
+    // left it: the mirror (vector_db_read) must show that state first
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      cv_applied_.wait(g, [&] { return mirror_applied_ >= t.mirror_seq; });
+    }
+    finish(t.reqs, f.t0, f.tick, t.nq);
+  }
+
+  void fail_tick(Tick& t, int rc, int64_t t0, uint64_t tick) {
+    for (auto& r : t.reqs) r->rc = rc;
+    finish(t.reqs, t0, tick, t.nq);
   }
 
   // Device half of a mutation (the engine store and its shadow).  Indices
