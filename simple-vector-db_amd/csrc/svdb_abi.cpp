@@ -15,12 +15,14 @@
 #include <algorithm>
 #include <cmath>
 #include <new>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "kdtree.h"
 #include "svdb.h"
 #include "svdb_engine.h"
+#include "svdb_mirror.h"
 #include "svdb_uuid.h"
 #include "vector_database.h"
 
@@ -404,8 +406,7 @@ void mirror_delete(VectorDatabase* db, size_t index) {
   if (index < db->size) {
     uuids_of(db)->erase(index, db->vectors[index].uuid, UUID_SIZE);
     free(db->vectors[index].data);
-    memmove(&db->vectors[index], &db->vectors[index + 1],
-            (db->size - index - 1) * sizeof(Vector));
+    svdb::shift_down_one(db->vectors, index, db->size);
     db->size--;
   }
   pthread_mutex_unlock(&db->mutex);
@@ -424,8 +425,7 @@ void vector_db_delete(VectorDatabase* db, size_t index) {
     } else {
       uuids_of(db)->erase(index, db->vectors[index].uuid, UUID_SIZE);
       free(db->vectors[index].data);
-      memmove(&db->vectors[index], &db->vectors[index + 1],
-              (db->size - index - 1) * sizeof(Vector));
+      svdb::shift_down_one(db->vectors, index, db->size);
       db->size--;
     }
   }

@@ -402,13 +402,38 @@ class Server {
       o += r->nq;
     }
     // the ids name rows of the store as the mutations before this tick
-    // left it: the mirror (vector_db_read) must show that state first
+    // left it: the mirror (vector_db_read) must show that state first.  If
+    // it does not yet, the mirror thread completes the requests once it
+    // does, and the executor goes on with the next tick.
     {
       std::unique_lock<std::mutex> g(mu_);
-      cv_applied_.wait(g, [&] { return mirror_applied_ >= t.mirror_seq; });
+      if (mirror_applied_ < t.mirror_seq) {
+        bool handed = false;
+        try {
+          fault_point();
+          after_mirror_.push_back(Waiting{t.mirror_seq, f.t0, f.tick, t.nq, std::move(t.reqs)});
+          handed = true;
+        } catch (...) {
+        }
+        if (handed) {
+          cv_mirror_.notify_all();
+          return;
+        }
+        // (no room to hand over: wait here instead)
+        cv_applied_.wait(g, [&] { return mirror_applied_ >= t.mirror_seq; });
+      }
     }
     finish(t.reqs, f.t0, f.tick, t.nq);
   }
+
+  // Search requests answered, waiting for the mirror to reach `seq`.
+  struct Waiting {
+    uint64_t seq;
+    int64_t t0;
+    uint64_t tick;
+    size_t nq;
+    std::vector<std::shared_ptr<Request>> reqs;
+  };
 
   void fail_tick(Tick& t, int rc, int64_t t0, uint64_t tick) {
     for (auto& r : t.reqs) r->rc = rc;
@@ -444,7 +469,17 @@ class Server {
       std::shared_ptr<Request> r;
       {
         std::unique_lock<std::mutex> g(mu_);
-        cv_mirror_.wait(g, [&] { return mirror_stop_ || !mirror_q_.empty(); });
+        cv_mirror_.wait(g, [&] {
+          return mirror_stop_ || !mirror_q_.empty() || ready_after_mirror();
+        });
+        if (ready_after_mirror()) {
+          // searches handed over by the executor whose mutations are mirrored
+          Waiting w = std::move(after_mirror_.front());
+          after_mirror_.pop_front();
+          g.unlock();
+          finish(w.reqs, w.t0, w.tick, w.nq);
+          continue;
+        }
         if (mirror_q_.empty()) return;
         r = mirror_q_.front();
         mirror_q_.pop_front();
@@ -471,6 +506,11 @@ class Server {
       cv_applied_.notify_all();
       finish(r, r->t_start, r->tick, 0);
     }
+  }
+
+  // (under mu_) the oldest handed-over search tick can complete
+  bool ready_after_mirror() const {
+    return !after_mirror_.empty() && after_mirror_.front().seq <= mirror_applied_;
   }
 
   // Completes requests (allocates nothing).
@@ -507,6 +547,7 @@ class Server {
   std::list<std::shared_ptr<Request>> mirror_q_;
   uint64_t mirror_issued_ = 0;   // mutations handed to the mirror thread
   uint64_t mirror_applied_ = 0;  // ... and applied by it
+  std::deque<Waiting> after_mirror_;  // answered searches waiting for the mirror
   bool mirror_stop_ = false;
   std::unique_ptr<Tick> staged_;
   std::unordered_map<int64_t, std::shared_ptr<Request>> all_;
