@@ -49,6 +49,48 @@ __device__ __forceinline__ uint32_t uniform32(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ uint64_t read_lane64(uint64_t v, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double read_lane_f64(double v, int l) {
+  return __longlong_as_double((long long)read_lane64((uint64_t)__double_as_longlong(v), l));
+}
+
+// Cosine rank keys -(dot / qx) of a lane's two rows (qx = ‖q‖·‖x‖, the
+// canonical operand order).  The correctly rounded division costs a dozen
+// instructions per (query, row), so it is skipped for the whole wavefront
+// when a cheap estimate shows that no lane's rows can enter the list whose
+// k-th rank key is tk (the keys are then +inf, which nothing inserts).  The
+// estimate dot·rcp(qx) is within 2^-20 relative of the quotient (v_rcp_f64
+// is good to about 2^-26, the product rounds once), so an estimated key
+// above the k-th key by more than 2^-19 of its magnitude ranks after it;
+// non-finite, zero or extreme operands and a list not yet full take the
+// exact path.
+__device__ __forceinline__ void cos_keys(double dot0, double qx0, double dot1, double qx1,
+                                         uint64_t tk, double& k0, double& k1) {
+  bool exact = true;
+  if (tk != ~0ull) {
+    const double thr = rank_value(tk);
+    const double e0 = -(dot0 * __builtin_amdgcn_rcp(qx0));
+    const double e1 = -(dot1 * __builtin_amdgcn_rcp(qx1));
+    // (qx far from the extremes: its reciprocal stays a normal number)
+    const bool skip0 = qx0 > 0x1p-1000 && qx0 < 0x1p1000 && isfinite(e0) &&
+                       e0 - thr > 0x1p-19 * fabs(e0);
+    const bool skip1 = qx1 > 0x1p-1000 && qx1 < 0x1p1000 && isfinite(e1) &&
+                       e1 - thr > 0x1p-19 * fabs(e1);
+    exact = !isfinite(thr) || __ballot(!(skip0 && skip1)) != 0;
+  }
+  if (exact) {
+    k0 = -(dot0 / qx0);
+    k1 = -(dot1 / qx1);
+  } else {
+    k0 = k1 = INFINITY;
+  }
+}
+
 // A sorted list of (key, slot) spread over a wavefront: entry e lives in
 // lane e % 64, register e / 64.  Unused entries hold the sentinel
 // (UINT64_MAX, UINT32_MAX), which is above every real entry (slots are
@@ -224,19 +266,35 @@ template <int METRIC>
 __device__ __forceinline__ void accum8(double (&acc)[kScanTQ][2], const double (&x0)[8],
                                        const double (&x1)[8], const double* __restrict__ Qg,
                                        size_t ldq, uint32_t c) {
-  // Queries in groups of 4: a group's 32 values sit in scalar registers,
-  // and its 8 (query, row) chains interleave, so each chain's next fma is 8
-  // instructions after its last (FP64 fma latency); every chain still
-  // advances in element order.
-  constexpr int G = 4;
+  // Queries in groups of G: a group's 8·G values are scalar loads, and the
+  // next group's loads are issued (into the other of two register sets)
+  // before this group's fmas, with scheduling barriers keeping it that way —
+  // left to itself the compiler hoists all 16 queries' loads to the top
+  // (256 scalar registers) and spills most of them.  A group's 2·G
+  // (query, row) chains interleave; every chain advances in element order.
+  constexpr int G = 2;
+  constexpr int NG = kScanTQ / G;
+  double qv[2][G][8];
 #pragma unroll
-  for (int j0 = 0; j0 < kScanTQ; j0 += G) {
+  for (int jj = 0; jj < G; jj++)
+#pragma unroll
+    for (int i = 0; i < 8; i++) qv[0][jj][i] = Qg[(size_t)jj * ldq + c + i];
+#pragma unroll
+  for (int g = 0; g < NG; g++) {
+    if (g + 1 < NG) {
+#pragma unroll
+      for (int jj = 0; jj < G; jj++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          qv[(g + 1) & 1][jj][i] = Qg[(size_t)((g + 1) * G + jj) * ldq + c + i];
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
 #pragma unroll
       for (int jj = 0; jj < G; jj++) {
-        const int j = j0 + jj;
-        double q = Qg[(size_t)j * ldq + c + i];
+        const int j = g * G + jj;
+        const double q = qv[g & 1][jj][i];
         if (METRIC == kL2) {
           double t0 = x0[i] - q, t1 = x1[i] - q;
           acc[j][0] = fma(t0, t0, acc[j][0]);
@@ -247,6 +305,7 @@ __device__ __forceinline__ void accum8(double (&acc)[kScanTQ][2], const double (
         }
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -271,19 +330,22 @@ k_scan_f64(ScanF64Args a) {
   const bool full_group = q0 + kScanTQ <= a.nq;
 
   WList<KPL> L[kScanTQ];
-  uint64_t tk[kScanTQ];
-  uint32_t ts[kScanTQ];
-  int kq[kScanTQ];
-  double qn[kScanTQ];
-#pragma unroll
-  for (int j = 0; j < kScanTQ; j++) {
-    L[j].init();
-    tk[j] = ~0ull;
-    ts[j] = ~0u;
-    uint32_t qi = min(q0 + j, qlast);
-    kq[j] = a.k[qi];
-    qn[j] = METRIC == kCos ? a.qn[qi] : 0.0;
+  // Per-query state, lane j holding query j's: the current k-th best
+  // (key, slot), k and (cosine) ‖q‖.  Kept in vector registers and read
+  // into scalars one query at a time in the epilogue: as arrays of
+  // wave-uniform values they needed about 100 scalar registers and the
+  // kernel spilled scalars throughout its main loop.
+  uint64_t tk_l = ~0ull;
+  uint32_t ts_l = ~0u;
+  int kq_l = 0;
+  double qn_l = 0.0;
+  if (lane < kScanTQ) {
+    const uint32_t qi = min(q0 + lane, qlast);
+    kq_l = a.k[qi];
+    qn_l = METRIC == kCos ? a.qn[qi] : 0.0;
   }
+#pragma unroll
+  for (int j = 0; j < kScanTQ; j++) L[j].init();
 
   for (uint32_t base = r_begin + wave * kScanRows; base < r_end;
        base += kWavesPerBlock * kScanRows) {
@@ -334,35 +396,43 @@ k_scan_f64(ScanF64Args a) {
     }
 #pragma unroll
     for (int j = 0; j < kScanTQ; j++) {
+      uint64_t tkj = read_lane64(tk_l, j);
+      uint32_t tsj = __builtin_amdgcn_readlane(ts_l, j);
+      const int kqj = __builtin_amdgcn_readlane(kq_l, j);
       double s0 = acc[j][0], s1 = acc[j][1];
       if (METRIC == kDot) {
         s0 = -s0;
         s1 = -s1;
       } else if (METRIC == kCos) {
-        s0 = -(s0 / (qn[j] * xn0));
-        s1 = -(s1 / (qn[j] * xn1));
+        const double qnj = read_lane_f64(qn_l, j);
+        cos_keys(s0, qnj * xn0, s1, qnj * xn1, tkj, s0, s1);
       }
       const uint64_t k0 = rank_bits(s0), k1 = rank_bits(s1);
       // slots past nq (a partial group) keep no list
       const bool jv = q0 + j < a.nq;
-      uint64_t m0 = __ballot(jv && v0 && lex_lt(k0, r0, tk[j], ts[j]));
-      uint64_t m1 = __ballot(jv && v1 && lex_lt(k1, r1, tk[j], ts[j]));
+      uint64_t m0 = __ballot(jv && v0 && lex_lt(k0, r0, tkj, tsj));
+      uint64_t m1 = __ballot(jv && v1 && lex_lt(k1, r1, tkj, tsj));
       // After each insertion the k-th entry may have moved: lanes that no
       // longer beat it drop out (their insertion would change nothing).
+      const bool moved = (m0 | m1) != 0;
       while (m0) {
         int src = __ffsll((long long)m0) - 1;
         m0 &= m0 - 1;
-        L[j].insert(uniform64(__shfl(k0, src)), base + src, kq[j], lane);
-        L[j].kth(kq[j], tk[j], ts[j]);
-        m0 &= __ballot(lex_lt(k0, r0, tk[j], ts[j]));
+        L[j].insert(uniform64(__shfl(k0, src)), base + src, kqj, lane);
+        L[j].kth(kqj, tkj, tsj);
+        m0 &= __ballot(lex_lt(k0, r0, tkj, tsj));
       }
-      m1 &= __ballot(lex_lt(k1, r1, tk[j], ts[j]));
+      m1 &= __ballot(lex_lt(k1, r1, tkj, tsj));
       while (m1) {
         int src = __ffsll((long long)m1) - 1;
         m1 &= m1 - 1;
-        L[j].insert(uniform64(__shfl(k1, src)), base + 64 + src, kq[j], lane);
-        L[j].kth(kq[j], tk[j], ts[j]);
-        m1 &= __ballot(lex_lt(k1, r1, tk[j], ts[j]));
+        L[j].insert(uniform64(__shfl(k1, src)), base + 64 + src, kqj, lane);
+        L[j].kth(kqj, tkj, tsj);
+        m1 &= __ballot(lex_lt(k1, r1, tkj, tsj));
+      }
+      if (moved && lane == j) {
+        tk_l = tkj;
+        ts_l = tsj;
       }
     }
   }
@@ -371,11 +441,12 @@ k_scan_f64(ScanF64Args a) {
 #pragma unroll
   for (int j = 0; j < kScanTQ; j++) {
     if (q0 + j >= a.nq) break;
+    const int kqj = __builtin_amdgcn_readlane(kq_l, j);
     size_t o = ((size_t)(q0 + j) * a.P + list) * a.kstride;
 #pragma unroll
     for (int p = 0; p < KPL; p++) {
       int e = p * 64 + lane;
-      if (e < kq[j]) {
+      if (e < kqj) {
         a.part_key[o + e] = L[j].k[p];
         a.part_slot[o + e] = L[j].s[p];
       }
