@@ -79,6 +79,21 @@ int svdb_search(VectorDatabase* db, const double* queries, size_t nq, size_t dim
 int svdb_search_device(VectorDatabase* db, const double* d_queries, size_t nq, size_t ldq,
                        size_t k, int metric, int precision, uint64_t* d_ids, double* d_scores);
 
+/* The FP64 form of svdb_search_device (the exact scan) that allocates
+ * nothing, host or device: every intermediate (k per query, query norms, the
+ * row slices' partial top-k lists) lives in a 64 MiB device array that the
+ * library holds on each GPU for the life of the process.  No counterpart in
+ * the reference: it is for callers that cannot allocate — code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Limits: nq <= 8192 and nq * k small enough for the array (the row slices
+ * shrink to fit; at 8 slices nq * k <= about 170,000); larger requests get
+ * SVDB_EINVAL ("... allocation-free scratch") and nothing runs.  Calls from
+ * any thread are serialised (they share the array); ids and scores equal
+ * svdb_search_device(..., SVDB_PREC_FP64, ...) bit for bit. */
+int svdb_search_device_noalloc(VectorDatabase* db, const double* d_queries, size_t nq,
+                               size_t ldq, size_t k, int metric, uint64_t* d_ids,
+                               double* d_scores);
+
 /* FP64 metric for npairs pairs of stored rows (logical indices), one 64-lane
  * wavefront per pair: lane l sums elements i ≡ l (mod 64) with fma, then an
  * xor butterfly.  Euclidean returns the distance. */

@@ -35,6 +35,7 @@ EXPORTS = [
     "svdb_server_start", "svdb_server_stop", "svdb_server_submit_search",
     "svdb_server_submit_insert", "svdb_server_submit_update", "svdb_server_submit_delete",
     "svdb_server_wait", "svdb_debug_fail_host_alloc", "svdb_nearest_request",
+    "svdb_search_device_noalloc",
 ]
 
 
@@ -115,6 +116,8 @@ def lib():
     L.svdb_search.argtypes = [DBP, vp, sz, sz, vp, vp, C.c_int, sz, vp, vp]
     L.svdb_search_device.restype = C.c_int
     L.svdb_search_device.argtypes = [DBP, vp, sz, sz, sz, C.c_int, C.c_int, vp, vp]
+    L.svdb_search_device_noalloc.restype = C.c_int
+    L.svdb_search_device_noalloc.argtypes = [DBP, vp, sz, sz, sz, C.c_int, vp, vp]
     L.svdb_pair_metrics.restype = C.c_int
     L.svdb_pair_metrics.argtypes = [DBP, vp, vp, sz, C.c_int, vp]
     L.svdb_bulk_insert.restype = C.c_int
