@@ -301,8 +301,13 @@ static void topk_emit(const topk_t* t, size_t k, int metric, uint64_t* ids, doub
     for (size_t j = 0; j < k; j++) {
         if (j < t->cnt) {
             ids[j] = t->id[j];
-            /* every NaN score is emitted as the canonical quiet NaN */
-            sc[j] = isnan(t->key[j]) ? NAN : metric == ORC_L2 ? sqrt(t->key[j]) : -t->key[j];
+            /* every NaN score is emitted as the canonical quiet NaN and
+             * every zero score as +0: the ranking folds -0 onto +0, so
+             * neither a NaN's payload nor a zero's sign carries meaning */
+            sc[j] = isnan(t->key[j])   ? NAN
+                    : metric == ORC_L2 ? sqrt(t->key[j])
+                    : t->key[j] == 0.0 ? 0.0
+                                       : -t->key[j];
         } else {
             ids[j] = UINT64_MAX;
             sc[j] = NAN;

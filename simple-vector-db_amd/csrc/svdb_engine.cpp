@@ -845,27 +845,37 @@ int Engine::run_exact(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, con
   const size_t nlist = (size_t)nq * nslices * kWavesPerBlock * kmax;
   if ((rc = scratch(&c_->s_pk_, &c_->s_pk_n_, std::max<size_t>(nlist, 1) * sizeof(uint64_t)))) return rc;
   if ((rc = scratch(&c_->s_ps_, &c_->s_ps_n_, std::max<size_t>(nlist, 1) * sizeof(uint32_t)))) return rc;
-  if ((rc = scratch(&c_->s_qn_, &c_->s_qn_n_, (size_t)nq * sizeof(double)))) return rc;
+  if ((rc = scratch(&c_->s_qn_, &c_->s_qn_n_, (size_t)nq * 3 * sizeof(double)))) return rc;
   return run_exact_with(dQ, ldq, nq, D, dk, metric, kmax, dids, dscores, ostride, nslices,
                         (uint64_t*)c_->s_pk_, (uint32_t*)c_->s_ps_, (double*)c_->s_qn_);
 }
 
 // The exact scan and merge into caller-provided scratch: partial lists
-// pk / ps of nq × nslices × kWavesPerBlock × kmax entries, query norms qn.
-// Allocates nothing.
+// pk / ps of nq × nslices × kWavesPerBlock × kmax entries, and 3·nq
+// doubles at qn: the query norms, their squares as fma chains, and the
+// per-query global bounds of the scan (ScanF64Args.qq, .gtk).  Allocates
+// nothing.
 int Engine::run_exact_with(const double* dQ, size_t ldq, uint32_t nq, uint32_t D, const int* dk,
                            int metric, int kmax, uint64_t* dids, double* dscores,
                            uint32_t ostride, uint32_t nslices, uint64_t* pk, uint32_t* ps,
                            double* qn) {
   int rc;
   if (nq == 0) return SVDB_OK;
-  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
+  // the squared row norms cover every column: usable when the scan does
+  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0,
+               D == d_ ? nx2_ : nullptr};
   const uint32_t rps = nslices ? (uint32_t)((nphys_ + nslices - 1) / nslices) : 0;
-  const uint32_t P = nslices * kWavesPerBlock;
+  const uint32_t P = scan_f64_lists(metric, D, sv, kmax, nslices);
+  const bool mfma = scan_f64_uses_mfma(metric, D, sv, kmax);
+  double* qq = qn + nq;
+  uint64_t* gtk = reinterpret_cast<uint64_t*>(qn + 2 * (size_t)nq);
   if ((rc = sync_alive())) return rc;
-  if (metric == kCos) HIPCHK(launch_query_norms(dQ, ldq, nq, D, qn, c_->stream), "k_query_norms");
+  if (metric == kCos || (metric == kL2 && mfma))
+    HIPCHK(launch_query_norms(dQ, ldq, nq, D, qn, metric == kL2 ? qq : nullptr, c_->stream),
+           "k_query_norms");
   if (nslices) {
-    ScanF64Args a{sv, dQ, ldq, qn, dk, nq, D, rps, P, (uint32_t)kmax, pk, ps};
+    HIPCHK(launch_fill_i32(reinterpret_cast<int*>(gtk), 2 * nq, -1, c_->stream), "k_fill_i32");
+    ScanF64Args a{sv, dQ, ldq, qn, dk, nq, D, rps, P, (uint32_t)kmax, pk, ps, gtk, qq};
     prof_begin(SVDB_K_SCAN_FP64);
     HIPCHK(launch_scan_f64(a, metric, kmax, nslices, c_->stream), "k_scan_f64");
     prof_end(SVDB_K_SCAN_FP64);
@@ -1135,7 +1145,7 @@ int Engine::lowp_enqueue(const double* dQ, size_t ldq, LowpRange* R, int nr, int
   prof_begin(SVDB_K_SHORTLIST);
   HIPCHK(launch_shortlist_ranges(sls.data(), nr, c_->stream), "k_shortlist");
   prof_end(SVDB_K_SHORTLIST);
-  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
+  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0, nullptr};
   const uint32_t Lmax = L;  // shortlist block stride per slot
   // FP64 rerank of every range's shortlist; fail counts and the first-pass
   // error report come back together after one wait
@@ -1209,7 +1219,7 @@ int Engine::lowp_complete() {
   auto w1_of = [&](int m) { return fp8 ? (m == kCos ? sh8_.pen : sh8_.sx) : sh_.pen; };
   const uint32_t* degen = fp8 ? sh8_.degen : sh_.degen;
   const uint32_t* degen_count = fp8 ? sh8_.degen_count : sh_.degen_count;
-  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0};
+  StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0, nullptr};
   const uint32_t Lmax = width(lp.k);  // shortlist block stride per slot
   auto rerank_args = [&](int ri) {
     const LowpRange& rg = R[ri];
@@ -1460,9 +1470,10 @@ int Engine::search_device_noalloc(const double* dQ, size_t nq, size_t ldq, size_
     return SVDB_EINVAL;
   }
   if (nq == 0) return SVDB_OK;
-  // scratch: k per query | query norms | partial keys | partial slots
+  // scratch: k per query | query norms, squared norms, global bounds |
+  // partial keys | partial slots
   char* base = static_cast<char*>(scratch);
-  const size_t kb = round_up(nq * sizeof(int), 256), qb = round_up(nq * sizeof(double), 256);
+  const size_t kb = round_up(nq * sizeof(int), 256), qb = round_up(nq * 3 * sizeof(double), 256);
   if (nq > kMaxBatch || kb + qb >= bytes) {
     set_error_static(SVDB_EINVAL, "batch too large for the allocation-free search");
     return SVDB_EINVAL;

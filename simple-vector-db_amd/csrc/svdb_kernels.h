@@ -24,6 +24,8 @@ struct StoreView {
   const uint32_t* rank_base;// alive slots before each 64-slot word
   uint32_t nrows;           // slots in use
   int has_deletes;          // 0: logical id == slot
+  const double* nx2;        // Σ fma(x_i,x_i,·) per row when the scan compares
+                            // every column, else null
 };
 
 struct ScanF64Args {
@@ -39,6 +41,12 @@ struct ScanF64Args {
   uint32_t kstride;         // entries per partial list (>= max k)
   uint64_t* part_key;       // [nq][P][kstride]
   uint32_t* part_slot;
+  // Per query, the smallest k-th rank key any partial list has reached so
+  // far (all ones before the scan; the matrix-core scan lowers it with
+  // atomic min and skips every row whose key is above it), or null.
+  uint64_t* gtk;
+  const double* qq;         // per-query Σ fma(q_i,q_i,·) (Euclidean on the
+                            // matrix cores), or null
 };
 
 struct MergeArgs {
@@ -59,8 +67,18 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_fill_i32(int* p, uint32_t n, int v, hipStream_t s);
 hipError_t launch_row_norms(const double* X, size_t ldx, const uint32_t* slots, uint32_t first,
                             uint32_t n, uint32_t d, double* nx2, double* xn, hipStream_t s);
+// qn[i] = sqrt(Σ fma(q,q,·)) over the first D columns; qq[i] the sum itself
+// when qq is not null.
 hipError_t launch_query_norms(const double* Q, size_t ldq, uint32_t nq, uint32_t D, double* qn,
+                              double* qq,
                               hipStream_t s);
+// Whether the FP64 scan of this metric, width and row layout runs on the
+// matrix cores, and how many partial lists per query it then writes for
+// nslices row slices (the value ScanF64Args.P and MergeArgs.P must hold).
+// (kmax: the largest k of the batch, ScanF64Args.kstride)
+bool scan_f64_uses_mfma(int metric, uint32_t D, const StoreView& st, int kmax);
+uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax,
+                        uint32_t nslices);
 hipError_t launch_scan_f64(const ScanF64Args& a, int metric, int kmax, uint32_t nslices,
                            hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, int metric, int kmax, hipStream_t s);

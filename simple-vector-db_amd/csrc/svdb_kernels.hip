@@ -194,19 +194,21 @@ hipError_t launch_row_norms(const double* X, size_t ldx, const uint32_t* slots, 
 }
 
 __global__ void k_query_norms(const double* __restrict__ Q, size_t ldq, uint32_t nq, uint32_t D,
-                              double* __restrict__ qn) {
+                              double* __restrict__ qn, double* __restrict__ qq) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
   const double* q = Q + (size_t)i * ldq;
   double acc = 0.0;
   for (uint32_t c = 0; c < D; c++) acc = fma(q[c], q[c], acc);
   qn[i] = sqrt(acc);
+  if (qq) qq[i] = acc;
 }
 
 hipError_t launch_query_norms(const double* Q, size_t ldq, uint32_t nq, uint32_t D, double* qn,
-                              hipStream_t s) {
+                              double* qq, hipStream_t s) {
   if (nq == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_query_norms, dim3((nq + 255) / 256), dim3(256), 0, s, Q, ldq, nq, D, qn);
+  hipLaunchKernelGGL(k_query_norms, dim3((nq + 255) / 256), dim3(256), 0, s, Q, ldq, nq, D, qn,
+                     qq);
   return hipGetLastError();
 }
 
@@ -394,24 +396,31 @@ k_scan_f64(ScanF64Args a) {
       xn0 = a.st.xn[c0];
       xn1 = a.st.xn[c1];
     }
+    // lane j: query q0 + j's global bound (the smallest k-th key any list
+    // of the query has published; rows with keys above it are out)
+    uint64_t gk_l = ~0ull;
+    if (a.gtk && lane < kScanTQ)
+      gk_l = __hip_atomic_load(a.gtk + min(q0 + lane, qlast), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int j = 0; j < kScanTQ; j++) {
       uint64_t tkj = read_lane64(tk_l, j);
       uint32_t tsj = __builtin_amdgcn_readlane(ts_l, j);
       const int kqj = __builtin_amdgcn_readlane(kq_l, j);
+      const uint64_t gkj = read_lane64(gk_l, j);
       double s0 = acc[j][0], s1 = acc[j][1];
       if (METRIC == kDot) {
         s0 = -s0;
         s1 = -s1;
       } else if (METRIC == kCos) {
         const double qnj = read_lane_f64(qn_l, j);
-        cos_keys(s0, qnj * xn0, s1, qnj * xn1, tkj, s0, s1);
+        cos_keys(s0, qnj * xn0, s1, qnj * xn1, min(tkj, gkj), s0, s1);
       }
       const uint64_t k0 = rank_bits(s0), k1 = rank_bits(s1);
       // slots past nq (a partial group) keep no list
       const bool jv = q0 + j < a.nq;
-      uint64_t m0 = __ballot(jv && v0 && lex_lt(k0, r0, tkj, tsj));
-      uint64_t m1 = __ballot(jv && v1 && lex_lt(k1, r1, tkj, tsj));
+      uint64_t m0 = __ballot(jv && v0 && k0 <= gkj && lex_lt(k0, r0, tkj, tsj));
+      uint64_t m1 = __ballot(jv && v1 && k1 <= gkj && lex_lt(k1, r1, tkj, tsj));
       // After each insertion the k-th entry may have moved: lanes that no
       // longer beat it drop out (their insertion would change nothing).
       const bool moved = (m0 | m1) != 0;
@@ -433,6 +442,9 @@ k_scan_f64(ScanF64Args a) {
       if (moved && lane == j) {
         tk_l = tkj;
         ts_l = tsj;
+        // a full list publishes its k-th key
+        if (a.gtk && tkj != ~0ull && tkj < gk_l)
+          __hip_atomic_fetch_min(a.gtk + q0 + j, tkj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }
@@ -454,10 +466,395 @@ k_scan_f64(ScanF64Args a) {
   }
 }
 
+// ------------------------------------------------- FP64 scan on MFMA
+//
+// The FP64 scan on the matrix cores.  v_mfma_f64_16x16x4f64 adds the four
+// products of a k-step onto each accumulator one after the other, each as
+// one correctly rounded fused multiply-add (the same bits as sequential
+// fma chains, subnormal, infinite and NaN operands included), so chained
+// over the k-steps in column order it computes exactly the canonical dot
+// key s = Σ_i fma(q_i, x_i, s); cosine divides it as the VALU scan does.
+// Euclidean keys, Σ fma(t, t, s) with t = x_i − q_i, have no matrix form:
+// the dot product gives a = (xx + qq) − 2·dot (xx, qq the canonical
+// squared norms), within E = c·(‖x‖ + ‖q‖)² + e of the canonical key
+// (l2_bound), and only a pair whose lower bound a − E could still enter
+// its query's list computes the canonical key, one fma chain per lane
+// (exact_l2); every other pair is provably outside the list.
+//
+// Block: 4 wavefronts, 16 queries each (64 per block), sharing the rows of
+// one slice, which stream through LDS in stages of 64 rows × 32 columns
+// (16 KB, global_load_lds_dwordx4, double-buffered, so the next stage's
+// loads run under this stage's MFMAs).  16-byte units are XOR-swizzled by
+// row so the operand reads of 16 rows hit distinct banks.  Lane l,
+// g = l / 16, j = l % 16; k-step s of a stage covers its columns
+// 4s..4s+3:
+//   A (queries)  lane l holds Q[qw + j][col0 + 4s + g]   (global loads)
+//   B (rows)     lane l holds X[row0 + 16t + j][col0 + 4s + g]  (LDS)
+//   accumulator  tile t, register r of lane l: query qw + g + 4r, row
+//                row0 + 16t + j
+// After a row step's last stage each (query, row) key is tested against
+// its query's k-th best (the wave-distributed lists of k_scan_f64) and
+// against the query's global bound gtk: the smallest k-th key any list of
+// the query has published (atomic min).  That list holds k rows with keys
+// at most gtk, so a row whose key is above gtk is not among the query's k
+// nearest; a key equal to it still goes to the list, whose (key, slot)
+// order settles the tie.  Lists then see far fewer insertions than a list
+// bounded by its own slice alone.
+// Grid: (groups of 64 queries) × slices, in k_scan_f64's XCD order; each
+// block writes one partial list per query (P = number of slices).
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void_t;
+constexpr int kF64StageRows = 64;
+constexpr int kF64StageCols = 32;
+constexpr int kF64StageBytes = kF64StageRows * kF64StageCols * 8;
+
+// Byte offset in a stage of 16-byte unit u (columns 2u, 2u+1) of row r.
+__device__ __forceinline__ int f64_unit(int r, int u) { return (r * 16 + (u ^ (r & 15))) * 16; }
+
+// The stage of rows row0.. (clamped to the store) and columns col0..col0+31:
+// 16 direct-to-LDS instructions of 64 × 16 bytes, four per wavefront.
+__device__ __forceinline__ void f64_issue_stage(const StoreView& st, unsigned char* buf,
+                                                uint32_t row0, uint32_t col0, int wave,
+                                                int lane) {
+#pragma unroll
+  for (int i = 0; i < kF64StageBytes / (1024 * kWavesPerBlock); i++) {
+    const int p0 = (wave + kWavesPerBlock * i) * 64;  // first unit slot of this instruction
+    const int p = p0 + lane;
+    const int r = p >> 4, u = (p & 15) ^ (r & 15);
+    const uint32_t row = min(row0 + (uint32_t)r, st.nrows - 1);
+    __builtin_amdgcn_global_load_lds(st.X + (size_t)row * st.ldx + col0 + 2 * u,
+                                     (lds_void_t*)(buf + p0 * 16), 16, 0, 0);
+  }
+}
+
+// The canonical Euclidean key of row x and query q over D columns
+// (D a multiple of 8, x 16-byte aligned): Σ_i fma(t, t, s), t = x_i − q_i.
+__device__ __forceinline__ double exact_l2(const double* __restrict__ x,
+                                          const double* __restrict__ q, uint32_t D) {
+  double s = 0.0;
+  for (uint32_t c = 0; c < D; c += 8) {
+    double xv[8], qv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      const double2 v = *reinterpret_cast<const double2*>(x + c + i);
+      xv[i] = v.x;
+      xv[i + 1] = v.y;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) qv[i] = q[c + i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const double t = xv[i] - qv[i];
+      s = fma(t, t, s);
+    }
+  }
+  return s;
+}
+
+// Bound on |a − s*| for a = fl(fl(xx + qq) − 2·dot) and the canonical key
+// s*, with xx, qq, dot and s* the canonical fma chains over D columns and
+// u = 2^-53.  Each chain is within γ_D = D·u/(1 − D·u) of its exact value
+// relative to its sum of |terms| (xx: ‖x‖², qq: ‖q‖², dot: ‖q‖‖x‖), the two
+// roundings of a add at most 2u(1 + γ_D)·S with S = (‖x‖ + ‖q‖)², and the
+// rounded differences t_i change ‖x − q‖² ≤ S by at most 2u + u², so
+// |a − s*| ≤ (2γ_D + 4u + O(u²))·S, and the final a − E rounds by at most
+// 2u·S more.  E = (4D + 128)·u·sn² with sn = fl(sqrt(xx)) + fl(sqrt(qq))
+// (≥ (‖x‖ + ‖q‖)(1 − γ_D − 3u)) is more than twice that for any D the
+// store allows, and e = (8D + 64)·2^-1074 covers the absolute error of
+// subnormal results (at most one 2^-1074 per rounding, 5D + 4 of them).
+// A non-finite operand gives a non-finite bound: the pair takes the
+// exact path.
+__device__ __forceinline__ double l2_lower(double xx, double qq, double dot, double xs,
+                                          double qs, double crel, double eabs) {
+  const double ap = (xx + qq) - 2.0 * dot;
+  const double sn = xs + qs;
+  const double E = crel * (sn * sn) + eabs;
+  return ap - E;
+}
+
+template <int METRIC, int KPL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+k_scan_f64_mfma(ScanF64Args a) {
+  constexpr int NT = kF64StageRows / 16;
+  constexpr int KS = kF64StageCols / 4;
+  __shared__ __attribute__((aligned(16))) unsigned char buf0[kF64StageBytes];
+  __shared__ __attribute__((aligned(16))) unsigned char buf1[kF64StageBytes];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, jr = lane & 15;
+  constexpr uint32_t QB = kScanTQ * kWavesPerBlock;
+  const uint32_t ngroups = (a.nq + QB - 1) / QB;
+  const uint32_t xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+  const uint32_t qw = (local % ngroups) * QB + wave * kScanTQ;  // this wave's first query
+  const uint32_t slice = (local / ngroups) * 8 + xcd;
+  const uint32_t r_begin = slice * a.rows_per_slice;
+  const uint32_t r_end = min(r_begin + a.rows_per_slice, a.st.nrows);
+  const uint32_t nsteps = r_end > r_begin ? (r_end - r_begin + kF64StageRows - 1) / kF64StageRows : 0;
+  const uint32_t nchunks = a.D / kF64StageCols;
+  const uint32_t nstages = nsteps * nchunks;
+  const uint32_t qlast = a.nq - 1;
+  const bool active = qw < a.nq;  // wave-uniform: a wave past nq only helps load
+  const double* __restrict__ qp = a.Q + (size_t)min(qw + jr, qlast) * a.ldq;
+
+  // Lane j holds query qw + j's current k-th best, k and norm.
+  uint64_t tk_l = ~0ull;
+  uint32_t ts_l = ~0u;
+  int kq_l = 0;
+  double qn_l = 0.0, qq_l = 0.0;
+  if (lane < kScanTQ) {
+    const uint32_t qi = min(qw + lane, qlast);
+    kq_l = a.k[qi];
+    qn_l = METRIC != kDot ? a.qn[qi] : 0.0;
+    if (METRIC == kL2) qq_l = a.qq[qi];
+  }
+  // the k-th best of this lane's accumulator queries g + 4r (filter)
+  uint64_t thk[4];
+  uint32_t ths[4];
+  double qn4[4], qq4[4];
+  bool qv4[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    thk[r] = ~0ull;
+    ths[r] = ~0u;
+    qn4[r] = __shfl(qn_l, g + 4 * r);
+    qq4[r] = __shfl(qq_l, g + 4 * r);
+    qv4[r] = qw + g + 4 * r < a.nq;
+  }
+  // Euclidean: the constants of l2_lower
+  const double crel = (4.0 * a.D + 128.0) * 0x1p-53;
+  const double eabs = (8.0 * a.D + 64.0) * 0x1p-1074;
+  WList<KPL> L[kScanTQ];
+#pragma unroll
+  for (int j = 0; j < kScanTQ; j++) L[j].init();
+  // lane j: query qw + j's global bound, read once per row step
+  uint64_t gk_l = ~0ull;
+
+  f64x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  double qa[KS], qn[KS];
+  auto load_q = [&](uint32_t col0) {
+#pragma unroll
+    for (int s = 0; s < KS; s++) qn[s] = qp[col0 + 4 * s + g];
+  };
+
+  // Per-row-step epilogue: keys of the step's 64 rows, list insertions.
+  auto epilogue = [&](uint32_t base) {
+    uint32_t rowi[NT], slot[NT];
+    bool rv[NT];
+    double xn[NT], xx[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      rowi[t] = base + 16 * t + jr;
+      slot[t] = min(rowi[t], a.st.nrows - 1);
+      rv[t] = rowi[t] < r_end && slot_alive(a.st.alive, slot[t]);
+      xn[t] = METRIC != kDot ? a.st.xn[slot[t]] : 0.0;
+      xx[t] = METRIC == kL2 ? a.st.nx2[slot[t]] : 0.0;
+    }
+    if (a.gtk && lane < kScanTQ)
+      gk_l = __hip_atomic_load(a.gtk + min(qw + lane, qlast), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t gk4[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) gk4[r] = __shfl(gk_l, g + 4 * r);
+    uint64_t key[NT][4];
+    bool hit = false;
+    uint32_t cm0 = 0xFFFFu;  // Euclidean: the pairs whose canonical key was computed
+    if constexpr (METRIC == kL2) {
+      // candidates: pairs whose lower bound could still enter the list
+      uint32_t cm = 0;
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const double lb = l2_lower(xx[t], qq4[r], acc[t][r], xn[t], qn4[r], crel, eabs);
+          const uint64_t kb = rank_bits(lb);
+          const bool c = qv4[r] && rv[t] &&
+                         (!(lb > -INFINITY && lb < INFINITY) || (kb <= thk[r] && kb <= gk4[r]));
+          cm |= (uint32_t)c << (t * 4 + r);
+          key[t][r] = ~0ull;
+        }
+      if (__builtin_expect(__ballot(cm != 0) == 0, 1)) return;
+      // the canonical key of each candidate, one pair per lane at a time
+      cm0 = cm;
+      while (cm) {
+        const int idx = __ffs(cm) - 1;
+        cm &= cm - 1;
+        const int t = idx >> 2, r = idx & 3;
+        const uint32_t srow = min(base + 16 * t + jr, a.st.nrows - 1);
+        const uint64_t kk =
+            rank_bits(exact_l2(a.st.X + (size_t)srow * a.st.ldx,
+                               a.Q + (size_t)min(qw + g + 4 * r, qlast) * a.ldq, a.D));
+#pragma unroll
+        for (int tt = 0; tt < NT; tt++)
+#pragma unroll
+          for (int rr = 0; rr < 4; rr++)
+            if (idx == tt * 4 + rr) key[tt][rr] = kk;
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          hit |= ((cm0 >> (t * 4 + r)) & 1) && key[t][r] <= gk4[r] &&
+                 lex_lt(key[t][r], rowi[t], thk[r], ths[r]);
+    } else {
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const double sc = METRIC == kCos ? -(acc[t][r] / (qn4[r] * xn[t])) : -acc[t][r];
+          key[t][r] = rank_bits(sc);
+          hit |= qv4[r] && rv[t] && key[t][r] <= gk4[r] &&
+                 lex_lt(key[t][r], rowi[t], thk[r], ths[r]);
+        }
+    }
+    if (__builtin_expect(__ballot(hit) == 0, 1)) return;
+#pragma unroll
+    for (int j = 0; j < kScanTQ; j++) {
+      const int r = j >> 2;
+      const bool mine = g == (j & 3);
+      if (qw + j >= a.nq) break;
+      uint64_t tkj = read_lane64(tk_l, j);
+      uint32_t tsj = __builtin_amdgcn_readlane(ts_l, j);
+      const int kqj = __builtin_amdgcn_readlane(kq_l, j);
+      const uint64_t gkj = read_lane64(gk_l, j);
+      bool moved = false;
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        uint64_t m = __ballot(mine && rv[t] && ((cm0 >> (t * 4 + r)) & 1) && key[t][r] <= gkj &&
+                              lex_lt(key[t][r], rowi[t], tkj, tsj));
+        while (m) {
+          const int src = __ffsll((long long)m) - 1;
+          m &= m - 1;
+          L[j].insert(uniform64(__shfl(key[t][r], src)), base + 16 * t + (src & 15), kqj, lane);
+          L[j].kth(kqj, tkj, tsj);
+          m &= __ballot(mine && lex_lt(key[t][r], rowi[t], tkj, tsj));
+          moved = true;
+        }
+      }
+      if (moved) {
+        if (lane == j) {
+          tk_l = tkj;
+          ts_l = tsj;
+          // a full list publishes its k-th key
+          if (a.gtk && tkj != ~0ull && tkj < gk_l)
+            __hip_atomic_fetch_min(a.gtk + qw + j, tkj, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (mine) {
+          thk[r] = tkj;
+          ths[r] = tsj;
+        }
+      }
+    }
+  };
+
+  if (nstages > 0) {
+    f64_issue_stage(a.st, buf0, r_begin, 0, wave, lane);
+    load_q(0);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  // One stage: issue the next stage's loads into `nxt`, MFMAs over `cur`,
+  // the row step's epilogue after its last stage, then wait for the loads
+  // and for every wave to be done with `cur`.  Stages alternate between the
+  // two buffers as distinct objects known at compile time, so reads of one
+  // need not wait for the direct-to-LDS loads still filling the other.
+  auto stage = [&](const unsigned char* cur, unsigned char* nxt, uint32_t st) {
+    const uint32_t step = st / nchunks, chunk = st % nchunks;
+#pragma unroll
+    for (int s = 0; s < KS; s++) qa[s] = qn[s];
+    if (st + 1 < nstages) {
+      const uint32_t nstep = (st + 1) / nchunks, nchunk = (st + 1) % nchunks;
+      f64_issue_stage(a.st, nxt, r_begin + nstep * kF64StageRows, nchunk * kF64StageCols, wave,
+                      lane);
+      load_q(nchunk * kF64StageCols);
+    }
+    if (active) {
+#pragma unroll
+      for (int s = 0; s < KS; s++) {
+        // B operand of tile t: row 16t + j, column 4s + g = unit 2s + g/2, half g&1
+        double b[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          b[t] = *reinterpret_cast<const double*>(cur + f64_unit(16 * t + jr, 2 * s + (g >> 1)) +
+                                                  (g & 1) * 8);
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[s], b[t], acc[t], 0, 0, 0);
+      }
+      if (chunk + 1 == nchunks) {
+        epilogue(r_begin + step * kF64StageRows);
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  };
+  for (uint32_t st0 = 0; st0 < nstages; st0 += 2) {
+    stage(buf0, buf1, st0);
+    if (st0 + 1 < nstages) stage(buf1, buf0, st0 + 1);
+  }
+
+  if (!active) return;
+  const uint32_t list = slice;
+#pragma unroll
+  for (int j = 0; j < kScanTQ; j++) {
+    if (qw + j >= a.nq) break;
+    const int kqj = __builtin_amdgcn_readlane(kq_l, j);
+    size_t o = ((size_t)(qw + j) * a.P + list) * a.kstride;
+#pragma unroll
+    for (int p = 0; p < KPL; p++) {
+      int e = p * 64 + lane;
+      if (e < kqj) {
+        a.part_key[o + e] = L[j].k[p];
+        a.part_slot[o + e] = L[j].s[p];
+      }
+    }
+  }
+}
+
+// Euclidean runs on the matrix cores up to k = 32: beyond it the lists
+// fill slowly enough that the pairs needing their canonical key (one fma
+// chain each) outweigh what the matrix cores save (1M × 128, 1,024
+// queries, k = 100: 164 ms against 31 ms on the VALU scan).
+bool scan_f64_uses_mfma(int metric, uint32_t D, const StoreView& st, int kmax) {
+  return D > 0 && D % kF64StageCols == 0 && st.ldx % 2 == 0 && ((uintptr_t)st.X & 15) == 0 &&
+         (metric != kL2 || (st.nx2 != nullptr && kmax <= 32));
+}
+
+// The matrix-core scan takes one list per (query, slice); up to k = 32 a
+// quarter of the slices, each four times longer (fewer lists fill fewer
+// times: 1M × 128, 1,024 queries, k = 10: dot 8.25 → 7.67 ms); at larger
+// k all of them (k = 100: 12.8 against 17.3 ms with the longer slices,
+// where the block's insertions dominate).
+uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax,
+                        uint32_t nslices) {
+  if (!scan_f64_uses_mfma(metric, D, st, kmax)) return nslices * kWavesPerBlock;
+  return kmax <= 32 ? std::max<uint32_t>(8, nslices / 32 * 8) : nslices;
+}
+
 template <int METRIC, int KPL>
 static hipError_t scan_f64_t(const ScanF64Args& a, uint32_t nslices, hipStream_t s) {
   // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid
   if (nslices % 8) return hipErrorInvalidValue;
+  if (a.P != scan_f64_lists(METRIC, a.D, a.st, (int)a.kstride, nslices))
+    return hipErrorInvalidValue;
+  // On the matrix cores when the width is whole stages and rows take
+  // 16-byte loads (Euclidean: all columns, with the squared norms at
+  // hand): one block per (64 queries, slice), one partial list per query
+  // and slice.
+  if (scan_f64_uses_mfma(METRIC, a.D, a.st, (int)a.kstride)) {
+    if (METRIC == kL2 && !a.qq) return hipErrorInvalidValue;
+    constexpr uint32_t QB = kScanTQ * kWavesPerBlock;
+    ScanF64Args m = a;
+    m.rows_per_slice = (a.st.nrows + a.P - 1) / a.P;
+    dim3 grid(((a.nq + QB - 1) / QB) * a.P);
+    hipLaunchKernelGGL((k_scan_f64_mfma<METRIC, KPL>), grid, dim3(256), 0, s, m);
+    return hipGetLastError();
+  }
   dim3 grid(((a.nq + kScanTQ - 1) / kScanTQ) * nslices);
   hipLaunchKernelGGL((k_scan_f64<METRIC, KPL>), grid, dim3(256), 0, s, a);
   return hipGetLastError();
