@@ -127,6 +127,8 @@ typedef struct svdb_stats {
     uint64_t second_pass;   /* FP8 queries handed to the BF16 first pass (counted again there) */
     uint64_t compactions;   /* tombstone compactions (live rows moved to the front, shadows
                                rebuilt) */
+    uint64_t graph_launches;/* exact searches run as one launch of a captured HIP graph */
+    uint64_t graph_captures;/* HIP graphs captured (a launch sequence seen a second time) */
 } svdb_stats;
 int svdb_get_stats(VectorDatabase* db, svdb_stats* out);
 

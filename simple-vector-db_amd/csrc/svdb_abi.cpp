@@ -758,6 +758,8 @@ int svdb_get_stats(VectorDatabase* db, svdb_stats* out) {
   out->widened = s.widened;
   out->second_pass = s.second_pass;
   out->compactions = s.compactions;
+  out->graph_launches = s.graph_launches;
+  out->graph_captures = s.graph_captures;
   return SVDB_OK;
 }
 

@@ -21,6 +21,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -66,6 +67,7 @@ struct KernelTime {
 struct EngineStats {
   uint64_t searches = 0, queries = 0, certified = 0, fallbacks = 0, widened = 0, second_pass = 0;
   uint64_t compactions = 0;
+  uint64_t graph_launches = 0, graph_captures = 0;
   double last_max_rel_err = 0.0;
 };
 
@@ -276,6 +278,15 @@ class Engine {
     double* scores = nullptr;
     uint64_t* pid = nullptr;   // pinned results (in h_stage_)
     double* psc = nullptr;
+    // Launch sequences of this context captured as HIP graphs, by key
+    // (run_graphed), and the keys run once eagerly so far.
+    struct Graph {
+      std::vector<uint64_t> key;
+      hipGraphExec_t exec = nullptr;
+      uint64_t used = 0;
+    };
+    std::vector<Graph> graphs;
+    std::vector<std::vector<uint64_t>> seen;
   };
   static constexpr int kCtx = 3;
   SearchCtx ctx_[kCtx];
@@ -291,6 +302,15 @@ class Engine {
                           size_t* ids, double* scores);
   int search_complete(SearchCtx& c);
   int complete_pending();  // completes every begun search (before a store mutation)
+  int run_graphed(SearchCtx& c, const std::vector<uint64_t>& key,
+                  const std::function<int()>& enqueue);
+  // Everything a captured launch sequence bakes in besides its key's own
+  // values: the store (rows, pointers, tombstones, shadows, shortlist) and
+  // the scratch and staging buffers.  Any change bumps one of these.
+  uint64_t store_ver_ = 0;
+  uint64_t scratch_gen_ = 0;
+  uint64_t graph_clock_ = 0;
+  bool graphs_on_ = false;  // SVDB_GRAPHS=1
 
   bool prof_ = false;
   bool prof_mute_ = false;  // inside the BF16 second pass: timed as a whole

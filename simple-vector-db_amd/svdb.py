@@ -60,7 +60,8 @@ class Stats(C.Structure):
                 ("searches", C.c_uint64), ("queries", C.c_uint64), ("certified", C.c_uint64),
                 ("fallbacks", C.c_uint64), ("last_max_rel_err", C.c_double),
                 ("widened", C.c_uint64), ("second_pass", C.c_uint64),
-                ("compactions", C.c_uint64)]
+                ("compactions", C.c_uint64), ("graph_launches", C.c_uint64),
+                ("graph_captures", C.c_uint64)]
 
 
 class KernelTime(C.Structure):
