@@ -173,7 +173,12 @@ size_t kdtree_nearest(KDTree* tree, const double* point) {
   size_t k = 1, id = (size_t)-1;
   int metric = SVDB_METRIC_L2;
   double score;
-  const int prec = D == e->dim() ? SVDB_PREC_AUTO : SVDB_PREC_FP64;
+  // One query: the low-precision pass carries ~100 µs of fixed work (query
+  // prep, pre-pass, shortlist, rerank, certification); the FP64 scan reads
+  // the compared columns of every row once, 512 MB at 2^26 elements, about
+  // as long.  Below that the FP64 scan answers (the same ids either way).
+  const bool small = (double)e->phys_rows() * (double)D <= (double)(1u << 26);
+  const int prec = D == e->dim() && !small ? SVDB_PREC_AUTO : SVDB_PREC_FP64;
   if (c_call([&] { return e->search_host(point, 1, D, D, &k, &metric, prec, 1, &id, &score); }) !=
       SVDB_OK) {
     fprintf(stderr, "kdtree_nearest: %s\n", svdb::last_error());

@@ -879,8 +879,8 @@ int Engine::run_exact_with(const double* dQ, size_t ldq, uint32_t nq, uint32_t D
   StoreView sv{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0,
                D == d_ ? nx2_ : nullptr};
   const uint32_t rps = nslices ? (uint32_t)((nphys_ + nslices - 1) / nslices) : 0;
-  const uint32_t P = scan_f64_lists(metric, D, sv, kmax, nslices);
-  const bool mfma = scan_f64_uses_mfma(metric, D, sv, kmax);
+  const uint32_t P = scan_f64_lists(metric, D, sv, kmax, nq, nslices);
+  const bool mfma = scan_f64_uses_mfma(metric, D, sv, kmax, nq);
   double* qq = qn + nq;
   uint64_t* gtk = reinterpret_cast<uint64_t*>(qn + 2 * (size_t)nq);
   if ((rc = sync_alive())) return rc;

@@ -75,9 +75,9 @@ hipError_t launch_query_norms(const double* Q, size_t ldq, uint32_t nq, uint32_t
 // Whether the FP64 scan of this metric, width and row layout runs on the
 // matrix cores, and how many partial lists per query it then writes for
 // nslices row slices (the value ScanF64Args.P and MergeArgs.P must hold).
-// (kmax: the largest k of the batch, ScanF64Args.kstride)
-bool scan_f64_uses_mfma(int metric, uint32_t D, const StoreView& st, int kmax);
-uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax,
+// (kmax: the largest k of the batch, ScanF64Args.kstride; nq its queries)
+bool scan_f64_uses_mfma(int metric, uint32_t D, const StoreView& st, int kmax, uint32_t nq);
+uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax, uint32_t nq,
                         uint32_t nslices);
 hipError_t launch_scan_f64(const ScanF64Args& a, int metric, int kmax, uint32_t nslices,
                            hipStream_t s);

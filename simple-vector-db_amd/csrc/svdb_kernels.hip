@@ -820,33 +820,42 @@ k_scan_f64_mfma(ScanF64Args a) {
 // fill slowly enough that the pairs needing their canonical key (one fma
 // chain each) outweigh what the matrix cores save (1M × 128, 1,024
 // queries, k = 100: 164 ms against 31 ms on the VALU scan).
-bool scan_f64_uses_mfma(int metric, uint32_t D, const StoreView& st, int kmax) {
+bool scan_f64_uses_mfma(int metric, uint32_t D, const StoreView& st, int kmax, uint32_t nq) {
+  (void)nq;
   return D > 0 && D % kF64StageCols == 0 && st.ldx % 2 == 0 && ((uintptr_t)st.X & 15) == 0 &&
          (metric != kL2 || (st.nx2 != nullptr && kmax <= 32));
 }
 
-// The matrix-core scan takes one list per (query, slice); up to k = 32 a
-// quarter of the slices, each four times longer (fewer lists fill fewer
-// times: 1M × 128, 1,024 queries, k = 10: dot 8.25 → 7.67 ms); at larger
-// k all of them (k = 100: 12.8 against 17.3 ms with the longer slices,
-// where the block's insertions dominate).
-uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax,
+// The matrix-core scan takes one list per (query, slice), with slices for
+// about 512 blocks up to k = 32 (two per CU, one round: fewer, longer lists
+// fill fewer times — 1M × 128, 1,024 queries, k = 10: 32 slices, dot 8.25
+// → 7.67 ms against 128) and about 1,024 above (k = 100: 12.7 ms at 64
+// slices, 13.5 at 128, 17.3 at 16 — the block's insertions dominating), at
+// least 64 rows a slice (a one-query call over 10,000 rows: 8 slices of
+// 1,250 rows took 193 µs per call, 64 slices 88 µs), and no more lists than
+// the scratch holds (nslices · kWavesPerBlock per query).
+uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax, uint32_t nq,
                         uint32_t nslices) {
-  if (!scan_f64_uses_mfma(metric, D, st, kmax)) return nslices * kWavesPerBlock;
-  return kmax <= 32 ? std::max<uint32_t>(8, nslices / 32 * 8) : nslices;
+  if (!scan_f64_uses_mfma(metric, D, st, kmax, nq)) return nslices * kWavesPerBlock;
+  const uint32_t groups = (nq + kScanTQ * kWavesPerBlock - 1) / (kScanTQ * kWavesPerBlock);
+  const uint32_t blocks = kmax <= 32 ? 512 : 1024;
+  uint32_t P = (blocks + groups - 1) / groups;
+  P = std::min<uint32_t>(P, (st.nrows + 63) / 64);
+  P = std::min<uint32_t>(P, nslices * kWavesPerBlock);
+  return std::max<uint32_t>(8, (P + 7) / 8 * 8);
 }
 
 template <int METRIC, int KPL>
 static hipError_t scan_f64_t(const ScanF64Args& a, uint32_t nslices, hipStream_t s) {
   // nslices is a multiple of 8 (one set of slices per XCD); 1-D grid
   if (nslices % 8) return hipErrorInvalidValue;
-  if (a.P != scan_f64_lists(METRIC, a.D, a.st, (int)a.kstride, nslices))
+  if (a.P != scan_f64_lists(METRIC, a.D, a.st, (int)a.kstride, a.nq, nslices))
     return hipErrorInvalidValue;
   // On the matrix cores when the width is whole stages and rows take
   // 16-byte loads (Euclidean: all columns, with the squared norms at
   // hand): one block per (64 queries, slice), one partial list per query
   // and slice.
-  if (scan_f64_uses_mfma(METRIC, a.D, a.st, (int)a.kstride)) {
+  if (scan_f64_uses_mfma(METRIC, a.D, a.st, (int)a.kstride, a.nq)) {
     if (METRIC == kL2 && !a.qq) return hipErrorInvalidValue;
     constexpr uint32_t QB = kScanTQ * kWavesPerBlock;
     ScanF64Args m = a;

@@ -9,7 +9,8 @@ at what that design could get wrong: non-finite and overflowing values
 (the bound turns non-finite), subnormal products, exact ties and zero
 distances, large offsets where (xx + qq) − 2·dot cancels almost every bit,
 deleted rows, stores shorter than one 64-row stage, partial query groups,
-and k on both sides of the Euclidean switch (32 / 33).
+and k on both sides of the Euclidean switch (32 / 33).  Every batch here
+holds at least 64 queries: smaller ones take the VALU scan.
 """
 import sys
 
@@ -84,8 +85,8 @@ def test_matrix_core_large_offset_near_ties():
     D = 64
     c = rng.standard_normal(D) * 1.25e5
     X = c + rng.standard_normal((20000, D)) * 1e-3
-    Q = c + rng.standard_normal((40, D)) * 1e-3
-    say("matrix-core scan, large offset near-ties, 20000x64, 40 queries")
+    Q = c + rng.standard_normal((80, D)) * 1e-3
+    say("matrix-core scan, large offset near-ties, 20000x64, 80 queries")
     db = svdb.Database(kd_dimension=D)
     db.bulk_insert(X)
     for metric in ("l2", "cos", "dot"):
@@ -99,7 +100,7 @@ def test_matrix_core_short_stores_and_deletes(n):
     rng = np.random.default_rng(100 + n)
     D = 32
     X = rng.standard_normal((n, D))
-    Q = rng.standard_normal((17, D))
+    Q = rng.standard_normal((70, D))          # (the matrix cores take batches of 64 and up)
     say(f"matrix-core scan, {n} rows")
     db = svdb.Database(kd_dimension=D)
     db.bulk_insert(X)
