@@ -172,15 +172,26 @@ def test_bf16_after_mutations_and_degenerate_rows():
     db.close()
 
 
-def test_kd_nearest_full_dimension_uses_first_pass(golden, c1_data):
-    # kd dimension == vector size: find-nearest goes through the BF16 first
-    # pass and must still equal the reference's own answers.
+def test_kd_nearest_full_dimension_paths(golden, c1_data):
+    # kd dimension == vector size.  A small store (rows x width up to 2^26)
+    # answers one query with the FP64 scan; a larger one through the BF16
+    # first pass; both must equal the reference's own answers / the oracle.
     X, Q, _ = c1_data
     db = svdb.Database(kd_dimension=128)
     db.bulk_insert(X)
     got = np.array([db.kd_nearest(q) for q in Q], dtype=np.uint64)
     assert np.array_equal(got, golden["c1_kd128_ids"])
-    assert db.stats()["certified"] > 200
+    assert db.stats()["certified"] == 0
+    db.close()
+    rng = np.random.default_rng(17)
+    Xl = rng.standard_normal((540_000, 128))  # 69.1M elements, above 2^26
+    Ql = rng.standard_normal((40, 128))
+    db = svdb.Database(kd_dimension=128)
+    db.bulk_insert(Xl)
+    got = np.array([db.kd_nearest(q) for q in Ql], dtype=np.uint64)
+    rid, _ = O.topk(Xl, Ql, 1, "l2")
+    assert np.array_equal(got, rid[:, 0])
+    assert db.stats()["certified"] >= 36
     db.close()
 
 
