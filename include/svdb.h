@@ -144,6 +144,14 @@ enum { SVDB_K_SCAN_FP64 = 0, SVDB_K_SCAN_LOWP = 1, SVDB_K_SHORTLIST = 2, SVDB_K_
 int svdb_set_profiling(VectorDatabase* db, int on); /* also resets the counters */
 int svdb_get_kernel_times(VectorDatabase* db, svdb_kernel_time* out /* [SVDB_K_COUNT] */);
 
+/* Measurement helper (no counterpart in the reference): the sustained
+ * matrix-core throughput of one data type on this device — a loop of
+ * independent MFMAs with no memory traffic at 2 and 4 wavefronts per SIMD,
+ * the best of three launches each — in TFLOP/s, the practical ceiling a
+ * scan's roofline fraction can be read against beside the published peak. */
+enum { SVDB_MFMA_FP64 = 0, SVDB_MFMA_BF16 = 1, SVDB_MFMA_FP8 = 2 };
+int svdb_mfma_sustained(int kind, double* tflops);
+
 /* ---- serving loop -------------------------------------------------------
  * A continuous-batching server over one store.  Searches and mutations are
  * submitted from any thread and answered in submission order by one device

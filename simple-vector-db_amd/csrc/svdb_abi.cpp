@@ -621,6 +621,60 @@ int svdb_device_copy(void* dst, const void* src, size_t bytes, int kind) {
   return SVDB_OK;
 }
 
+int svdb_mfma_sustained(int kind, double* tflops) {
+  if (!tflops || kind < SVDB_MFMA_FP64 || kind > SVDB_MFMA_FP8) {
+    svdb::set_error(SVDB_EINVAL, "svdb_mfma_sustained: kind or output");
+    return SVDB_EINVAL;
+  }
+  if (svdb::device_init() < 0) return SVDB_EDEVICE;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0) {
+    svdb::set_error(SVDB_EDEVICE, "svdb_mfma_sustained: device attributes");
+    return SVDB_EDEVICE;
+  }
+  // a few milliseconds a launch at the rates these loops reach (shorter
+  // launches read low while the clocks ramp)
+  const int iters = kind == SVDB_MFMA_FP64 ? 4096 : 16384;
+  hipStream_t st = nullptr;
+  float* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const uint32_t max_blocks = (uint32_t)cus * 4;
+  int rc = SVDB_OK;
+  double best = 0.0;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&out, (size_t)max_blocks * 256 * sizeof(float)) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    svdb::set_error(SVDB_EDEVICE, "svdb_mfma_sustained: setup");
+    rc = SVDB_EDEVICE;
+  }
+  for (uint32_t waves = 2; rc == SVDB_OK && waves <= 4; waves += 2) {
+    const uint32_t blocks = (uint32_t)cus * waves;  // 4 wavefronts a block, one per SIMD
+    if (svdb::launch_mfma_loop(kind, blocks, iters, out, st) != hipSuccess) rc = SVDB_EDEVICE;
+    for (int rep = 0; rc == SVDB_OK && rep < 3; rep++) {
+      float ms = 0.f;
+      if (hipEventRecord(e0, st) != hipSuccess ||
+          svdb::launch_mfma_loop(kind, blocks, iters, out, st) != hipSuccess ||
+          hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.f) {
+        rc = SVDB_EDEVICE;
+        break;
+      }
+      best = std::max(best, svdb::mfma_loop_flop(kind, blocks, iters) / (ms * 1e-3) / 1e12);
+    }
+  }
+  if (rc != SVDB_OK && svdb::last_error()[0] == 0)
+    svdb::set_error(SVDB_EDEVICE, "svdb_mfma_sustained: launch");
+  if (st) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (out) (void)hipFree(out);
+  if (st) (void)hipStreamDestroy(st);
+  *tflops = best;
+  return rc;
+}
+
 int svdb_device_sync(void) {
   if (svdb::device_init() < 0) return SVDB_EDEVICE;
   return hipDeviceSynchronize() == hipSuccess ? SVDB_OK : SVDB_EDEVICE;

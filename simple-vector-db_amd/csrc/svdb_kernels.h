@@ -82,6 +82,11 @@ uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax, u
 hipError_t launch_scan_f64(const ScanF64Args& a, int metric, int kmax, uint32_t nslices,
                            hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, int metric, int kmax, hipStream_t s);
+// A loop of independent MFMAs (kind 0 FP64 16x16x4, 1 BF16 16x16x32, 2 FP8
+// 16x16x128) over `blocks` blocks of 4 wavefronts; out holds blocks · 256
+// floats.  mfma_loop_flop gives the flop of one such launch.
+hipError_t launch_mfma_loop(int kind, uint32_t blocks, int iters, float* out, hipStream_t s);
+double mfma_loop_flop(int kind, uint32_t blocks, int iters);
 hipError_t launch_pair_f64(const double* X, size_t ldx, const uint32_t* sa, const uint32_t* sb,
                            uint32_t np, uint32_t D, int metric, double* out, hipStream_t s);
 hipError_t launch_pair_ref32(const double* M, size_t ld, const uint32_t* sa, const uint32_t* sb,

@@ -35,7 +35,7 @@ EXPORTS = [
     "svdb_server_start", "svdb_server_stop", "svdb_server_submit_search",
     "svdb_server_submit_insert", "svdb_server_submit_update", "svdb_server_submit_delete",
     "svdb_server_wait", "svdb_debug_fail_host_alloc", "svdb_nearest_request",
-    "svdb_search_device_noalloc",
+    "svdb_search_device_noalloc", "svdb_mfma_sustained",
 ]
 
 
@@ -281,6 +281,17 @@ class DeviceArray:
 
 def device_sync():
     _chk(lib().svdb_device_sync(), "svdb_device_sync")
+
+
+MFMA_KINDS = {"fp64": 0, "bf16": 1, "fp8": 2}
+
+
+def mfma_sustained(kind):
+    """Sustained matrix-core TFLOP/s of one data type on this device
+    (svdb_mfma_sustained): a loop of independent MFMAs, no memory traffic."""
+    v = C.c_double()
+    _chk(lib().svdb_mfma_sustained(MFMA_KINDS[kind], C.byref(v)), "svdb_mfma_sustained")
+    return v.value
 
 
 def legacy_metric(a, b, metric):
