@@ -423,6 +423,22 @@ __global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
     }
   };
 
+  // The previous subtile's accumulators and per-row values: its epilogue
+  // runs after the next subtile's first MFMAs have issued, so the key and
+  // list work (VALU) overlaps MFMAs in flight instead of leaving the
+  // matrix core idle — with one wavefront per SIMD nothing else covers it.
+  // The initial values make every key +inf (Euclidean: P > 0 with w0 =
+  // +inf; dot: P < 0 with w0 = -inf; cosine: w1 = +inf), so the first
+  // epilogue inserts nothing.
+  f32x4 pacc[QF];
+#pragma unroll
+  for (int f = 0; f < QF; f++) pacc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const float w0_init = COS ? 0.f : (METRIC == kDot ? -INFINITY : INFINITY);
+  float4 pw0 = make_float4(w0_init, w0_init, w0_init, w0_init);
+  const float w1_init = COS ? INFINITY : 1.f;
+  float4 pw1 = make_float4(w1_init, w1_init, w1_init, w1_init);
+  uint32_t prow = 0;
+
   if (ntiles > 0) issue_tile8<KS, NW>(a, buf0, r0, wave, lane);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -481,16 +497,23 @@ __global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
             Ab[ks & 1] = loadA(arow, ks + 2);
           }
           __builtin_amdgcn_sched_barrier(0);
+          if (ks == 0) {
+            epilogue(pacc, pw0, pw1, prow);
+            __builtin_amdgcn_sched_barrier(0);
+          }
         }
         const int rl = sub * 16 + rg * 4;
-        const float4 w0 = *reinterpret_cast<const float4*>(aux + rl);
-        const float4 w1 = *reinterpret_cast<const float4*>(aux + G::TR + rl);
-        epilogue(acc, w0, w1, trow + rl);
+#pragma unroll
+        for (int f = 0; f < QF; f++) pacc[f] = acc[f];
+        pw0 = *reinterpret_cast<const float4*>(aux + rl);
+        pw1 = *reinterpret_cast<const float4*>(aux + G::TR + rl);
+        prow = trow + rl;
       }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
     }
   }
+  if (active) epilogue(pacc, pw0, pw1, prow);
   const uint32_t list = slice * kLowpLanesPerQuery + rg;
 #pragma unroll
   for (int f = 0; f < QF; f++) {
