@@ -478,8 +478,11 @@ k_scan_f64(ScanF64Args a) {
 // the dot product gives a = (xx + qq) − 2·dot (xx, qq the canonical
 // squared norms), within E = c·(‖x‖ + ‖q‖)² + e of the canonical key
 // (l2_bound), and only a pair whose lower bound a − E could still enter
-// its query's list computes the canonical key, one fma chain per lane
-// (exact_l2); every other pair is provably outside the list.
+// its query's list computes the canonical key (exact_l2); every other pair
+// is provably outside the list.  Candidates wait in a per-wavefront LDS
+// queue and are keyed 64 at a time, one pair per lane (a lane at a time
+// had left the wavefront serial in those chains: Euclidean k = 10 10.4 →
+// 7.3 ms, k = 100 142 → 16.7 ms on 1M × 128 for 1,024 queries).
 //
 // Block: 4 wavefronts, 16 queries each (64 per block), sharing the rows of
 // one slice, which stream through LDS in stages of 64 rows × 32 columns
@@ -580,6 +583,10 @@ k_scan_f64_mfma(ScanF64Args a) {
   constexpr int KS = kF64StageCols / 4;
   __shared__ __attribute__((aligned(16))) unsigned char buf0[kF64StageBytes];
   __shared__ __attribute__((aligned(16))) unsigned char buf1[kF64StageBytes];
+  // Euclidean: each wavefront's queue of candidate pairs (slot << 8 | query
+  // of the wavefront), up to a row step's 1,024 on top of a part batch
+  constexpr int QCAP = METRIC == kL2 ? 64 + 16 * 16 * NT : 1;
+  __shared__ uint64_t cq[kWavesPerBlock][QCAP];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, jr = lane & 15;
@@ -639,8 +646,50 @@ k_scan_f64_mfma(ScanF64Args a) {
     for (int s = 0; s < KS; s++) qn[s] = qp[col0 + 4 * s + g];
   };
 
+  // Euclidean: n queued pairs from cq[wave][start], one per lane: the
+  // canonical key, then the lists (as the dot / cosine insertion below,
+  // with each lane's query read from its entry).
+  uint32_t qcount = 0;
+  auto run_batch = [&](uint32_t start, uint32_t n) __attribute__((always_inline)) {
+    const bool v = (uint32_t)lane < n;
+    const uint64_t e = v ? cq[wave][start + lane] : 0ull;
+    const uint32_t srow = (uint32_t)(e >> 8);
+    const int jq = (int)(e & 255u);
+    uint64_t kk = ~0ull;
+    if (v)
+      kk = rank_bits(exact_l2(a.st.X + (size_t)srow * a.st.ldx,
+                              a.Q + (size_t)min(qw + (uint32_t)jq, qlast) * a.ldq, a.D));
+#pragma unroll
+    for (int j = 0; j < kScanTQ; j++) {
+      if (qw + j >= a.nq) break;
+      uint64_t tkj = read_lane64(tk_l, j);
+      uint32_t tsj = __builtin_amdgcn_readlane(ts_l, j);
+      const int kqj = __builtin_amdgcn_readlane(kq_l, j);
+      const uint64_t gkj = read_lane64(gk_l, j);
+      uint64_t m = __ballot(v && jq == j && kk <= gkj && lex_lt(kk, srow, tkj, tsj));
+      if (!m) continue;
+      while (m) {
+        const int src = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        L[j].insert(uniform64(__shfl(kk, src)), uniform32(__shfl(srow, src)), kqj, lane);
+        L[j].kth(kqj, tkj, tsj);
+        m &= __ballot(v && jq == j && lex_lt(kk, srow, tkj, tsj));
+      }
+      if (lane == j) {
+        tk_l = tkj;
+        ts_l = tsj;
+        if (a.gtk && tkj != ~0ull && tkj < gk_l)
+          __hip_atomic_fetch_min(a.gtk + qw + j, tkj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (g == (j & 3)) {
+        thk[j >> 2] = tkj;
+        ths[j >> 2] = tsj;
+      }
+    }
+  };
+
   // Per-row-step epilogue: keys of the step's 64 rows, list insertions.
-  auto epilogue = [&](uint32_t base) {
+  auto epilogue = [&](uint32_t base) __attribute__((always_inline)) {
     uint32_t rowi[NT], slot[NT];
     bool rv[NT];
     double xn[NT], xx[NT];
@@ -660,10 +709,10 @@ k_scan_f64_mfma(ScanF64Args a) {
     for (int r = 0; r < 4; r++) gk4[r] = __shfl(gk_l, g + 4 * r);
     uint64_t key[NT][4];
     bool hit = false;
-    uint32_t cm0 = 0xFFFFu;  // Euclidean: the pairs whose canonical key was computed
     if constexpr (METRIC == kL2) {
-      // candidates: pairs whose lower bound could still enter the list
-      uint32_t cm = 0;
+      // Candidates — pairs whose lower bound could still enter the list —
+      // join the wavefront's queue; full batches of 64 get their canonical
+      // keys one pair per lane, all lanes at once, and go to the lists.
 #pragma unroll
       for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -672,43 +721,29 @@ k_scan_f64_mfma(ScanF64Args a) {
           const uint64_t kb = rank_bits(lb);
           const bool c = qv4[r] && rv[t] &&
                          (!(lb > -INFINITY && lb < INFINITY) || (kb <= thk[r] && kb <= gk4[r]));
-          cm |= (uint32_t)c << (t * 4 + r);
-          key[t][r] = ~0ull;
+          const uint64_t m = __ballot(c);
+          if (m) {
+            if (c)
+              cq[wave][qcount + __popcll(m & ((1ull << lane) - 1ull))] =
+                  ((uint64_t)rowi[t] << 8) | (uint64_t)(g + 4 * r);
+            qcount += __popcll(m);
+          }
         }
-      if (__builtin_expect(__ballot(cm != 0) == 0, 1)) return;
-      // the canonical key of each candidate, one pair per lane at a time
-      cm0 = cm;
-      while (cm) {
-        const int idx = __ffs(cm) - 1;
-        cm &= cm - 1;
-        const int t = idx >> 2, r = idx & 3;
-        const uint32_t srow = min(base + 16 * t + jr, a.st.nrows - 1);
-        const uint64_t kk =
-            rank_bits(exact_l2(a.st.X + (size_t)srow * a.st.ldx,
-                               a.Q + (size_t)min(qw + g + 4 * r, qlast) * a.ldq, a.D));
-#pragma unroll
-        for (int tt = 0; tt < NT; tt++)
-#pragma unroll
-          for (int rr = 0; rr < 4; rr++)
-            if (idx == tt * 4 + rr) key[tt][rr] = kk;
+      while (qcount >= 64) {
+        qcount -= 64;
+        run_batch(qcount, 64);
       }
-#pragma unroll
-      for (int t = 0; t < NT; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          hit |= ((cm0 >> (t * 4 + r)) & 1) && key[t][r] <= gk4[r] &&
-                 lex_lt(key[t][r], rowi[t], thk[r], ths[r]);
-    } else {
-#pragma unroll
-      for (int t = 0; t < NT; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const double sc = METRIC == kCos ? -(acc[t][r] / (qn4[r] * xn[t])) : -acc[t][r];
-          key[t][r] = rank_bits(sc);
-          hit |= qv4[r] && rv[t] && key[t][r] <= gk4[r] &&
-                 lex_lt(key[t][r], rowi[t], thk[r], ths[r]);
-        }
+      return;
     }
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double sc = METRIC == kCos ? -(acc[t][r] / (qn4[r] * xn[t])) : -acc[t][r];
+        key[t][r] = rank_bits(sc);
+        hit |= qv4[r] && rv[t] && key[t][r] <= gk4[r] &&
+               lex_lt(key[t][r], rowi[t], thk[r], ths[r]);
+      }
     if (__builtin_expect(__ballot(hit) == 0, 1)) return;
 #pragma unroll
     for (int j = 0; j < kScanTQ; j++) {
@@ -722,7 +757,7 @@ k_scan_f64_mfma(ScanF64Args a) {
       bool moved = false;
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        uint64_t m = __ballot(mine && rv[t] && ((cm0 >> (t * 4 + r)) & 1) && key[t][r] <= gkj &&
+        uint64_t m = __ballot(mine && rv[t] && key[t][r] <= gkj &&
                               lex_lt(key[t][r], rowi[t], tkj, tsj));
         while (m) {
           const int src = __ffsll((long long)m) - 1;
@@ -761,7 +796,8 @@ k_scan_f64_mfma(ScanF64Args a) {
   // and for every wave to be done with `cur`.  Stages alternate between the
   // two buffers as distinct objects known at compile time, so reads of one
   // need not wait for the direct-to-LDS loads still filling the other.
-  auto stage = [&](const unsigned char* cur, unsigned char* nxt, uint32_t st) {
+  auto stage = [&](const unsigned char* cur, unsigned char* nxt, uint32_t st)
+                   __attribute__((always_inline)) {
     const uint32_t step = st / nchunks, chunk = st % nchunks;
 #pragma unroll
     for (int s = 0; s < KS; s++) qa[s] = qn[s];
@@ -797,6 +833,9 @@ k_scan_f64_mfma(ScanF64Args a) {
     stage(buf0, buf1, st0);
     if (st0 + 1 < nstages) stage(buf1, buf0, st0 + 1);
   }
+  if constexpr (METRIC == kL2) {
+    if (active && qcount) run_batch(0, qcount);
+  }
 
   if (!active) return;
   const uint32_t list = slice;
@@ -816,14 +855,14 @@ k_scan_f64_mfma(ScanF64Args a) {
   }
 }
 
-// Euclidean runs on the matrix cores up to k = 32: beyond it the lists
-// fill slowly enough that the pairs needing their canonical key (one fma
-// chain each) outweigh what the matrix cores save (1M × 128, 1,024
-// queries, k = 100: 164 ms against 31 ms on the VALU scan).
+// Euclidean needs the squared row norms over the compared columns (all of
+// them); a column prefix (kd-tree dimension below the row width) stays on
+// the VALU scan, as do widths that are not whole stages.
 bool scan_f64_uses_mfma(int metric, uint32_t D, const StoreView& st, int kmax, uint32_t nq) {
   (void)nq;
+  (void)kmax;
   return D > 0 && D % kF64StageCols == 0 && st.ldx % 2 == 0 && ((uintptr_t)st.X & 15) == 0 &&
-         (metric != kL2 || (st.nx2 != nullptr && kmax <= 32));
+         (metric != kL2 || st.nx2 != nullptr);
 }
 
 // The matrix-core scan takes one list per (query, slice), with slices for

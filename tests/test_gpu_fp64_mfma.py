@@ -9,7 +9,7 @@ at what that design could get wrong: non-finite and overflowing values
 (the bound turns non-finite), subnormal products, exact ties and zero
 distances, large offsets where (xx + qq) − 2·dot cancels almost every bit,
 deleted rows, stores shorter than one 64-row stage, partial query groups,
-and k on both sides of the Euclidean switch (32 / 33).  Every batch here
+and k on both sides of 64 (one or two list registers per lane).  Every batch here
 holds at least 64 queries: smaller ones take the VALU scan.
 """
 import sys
