@@ -583,10 +583,13 @@ k_scan_f64_mfma(ScanF64Args a) {
   constexpr int KS = kF64StageCols / 4;
   __shared__ __attribute__((aligned(16))) unsigned char buf0[kF64StageBytes];
   __shared__ __attribute__((aligned(16))) unsigned char buf1[kF64StageBytes];
-  // Euclidean: each wavefront's queue of candidate pairs (slot << 8 | query
-  // of the wavefront), up to a row step's 1,024 on top of a part batch
-  constexpr int QCAP = METRIC == kL2 ? 64 + 16 * 16 * NT : 1;
+  // Each wavefront's queue of candidate pairs (slot << 8 | query of the
+  // wavefront; dot and cosine also the key): Euclidean queues a whole row
+  // step (up to 1,024) on top of a part batch, dot and cosine (k <= 64)
+  // drain it whenever it holds a batch, so 127 entries at most
+  constexpr int QCAP = METRIC == kL2 ? 64 + 16 * 16 * NT : (KPL == 1 ? 128 : 1);
   __shared__ uint64_t cq[kWavesPerBlock][QCAP];
+  __shared__ uint64_t cqk[kWavesPerBlock][METRIC == kL2 ? 1 : QCAP];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, jr = lane & 15;
@@ -646,9 +649,10 @@ k_scan_f64_mfma(ScanF64Args a) {
     for (int s = 0; s < KS; s++) qn[s] = qp[col0 + 4 * s + g];
   };
 
-  // Euclidean: n queued pairs from cq[wave][start], one per lane: the
-  // canonical key, then the lists (as the dot / cosine insertion below,
-  // with each lane's query read from its entry).
+  // n queued pairs from cq[wave][start], one per lane: the key (Euclidean:
+  // the canonical fma chain; dot and cosine: queued with the pair), then
+  // the lists — for each query, the lanes whose pair beats its k-th best
+  // insert one at a time, the k-th best re-read after each.
   uint32_t qcount = 0;
   auto run_batch = [&](uint32_t start, uint32_t n) __attribute__((always_inline)) {
     const bool v = (uint32_t)lane < n;
@@ -656,9 +660,13 @@ k_scan_f64_mfma(ScanF64Args a) {
     const uint32_t srow = (uint32_t)(e >> 8);
     const int jq = (int)(e & 255u);
     uint64_t kk = ~0ull;
-    if (v)
-      kk = rank_bits(exact_l2(a.st.X + (size_t)srow * a.st.ldx,
-                              a.Q + (size_t)min(qw + (uint32_t)jq, qlast) * a.ldq, a.D));
+    if constexpr (METRIC == kL2) {
+      if (v)
+        kk = rank_bits(exact_l2(a.st.X + (size_t)srow * a.st.ldx,
+                                a.Q + (size_t)min(qw + (uint32_t)jq, qlast) * a.ldq, a.D));
+    } else {
+      if (v) kk = cqk[wave][start + lane];
+    }
 #pragma unroll
     for (int j = 0; j < kScanTQ; j++) {
       if (qw + j >= a.nq) break;
@@ -707,8 +715,6 @@ k_scan_f64_mfma(ScanF64Args a) {
     uint64_t gk4[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) gk4[r] = __shfl(gk_l, g + 4 * r);
-    uint64_t key[NT][4];
-    bool hit = false;
     if constexpr (METRIC == kL2) {
       // Candidates — pairs whose lower bound could still enter the list —
       // join the wavefront's queue; full batches of 64 get their canonical
@@ -735,51 +741,84 @@ k_scan_f64_mfma(ScanF64Args a) {
       }
       return;
     }
+    // dot and cosine: the keys are exact already.  With one list register
+    // per lane (k <= 64), pairs that beat their query's k-th best (and its
+    // global bound) join the queue, drained a batch of 64 at a time (1M ×
+    // 128, 1,024 queries, k = 10: dot 7.8 → 6.6 ms, cosine 8.1 → 7.2); with
+    // two, the registers the queue path needs cost spills and the pairs go
+    // to the lists directly (k = 100: 12.8 ms against 13.8 queued).
+    if constexpr (KPL == 1) {
 #pragma unroll
-    for (int t = 0; t < NT; t++)
+      for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const double sc = METRIC == kCos ? -(acc[t][r] / (qn4[r] * xn[t])) : -acc[t][r];
-        key[t][r] = rank_bits(sc);
-        hit |= qv4[r] && rv[t] && key[t][r] <= gk4[r] &&
-               lex_lt(key[t][r], rowi[t], thk[r], ths[r]);
-      }
-    if (__builtin_expect(__ballot(hit) == 0, 1)) return;
-#pragma unroll
-    for (int j = 0; j < kScanTQ; j++) {
-      const int r = j >> 2;
-      const bool mine = g == (j & 3);
-      if (qw + j >= a.nq) break;
-      uint64_t tkj = read_lane64(tk_l, j);
-      uint32_t tsj = __builtin_amdgcn_readlane(ts_l, j);
-      const int kqj = __builtin_amdgcn_readlane(kq_l, j);
-      const uint64_t gkj = read_lane64(gk_l, j);
-      bool moved = false;
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        uint64_t m = __ballot(mine && rv[t] && key[t][r] <= gkj &&
-                              lex_lt(key[t][r], rowi[t], tkj, tsj));
-        while (m) {
-          const int src = __ffsll((long long)m) - 1;
-          m &= m - 1;
-          L[j].insert(uniform64(__shfl(key[t][r], src)), base + 16 * t + (src & 15), kqj, lane);
-          L[j].kth(kqj, tkj, tsj);
-          m &= __ballot(mine && lex_lt(key[t][r], rowi[t], tkj, tsj));
-          moved = true;
+        for (int r = 0; r < 4; r++) {
+          const double sc = METRIC == kCos ? -(acc[t][r] / (qn4[r] * xn[t])) : -acc[t][r];
+          const uint64_t kt = rank_bits(sc);
+          const bool c = qv4[r] && rv[t] && kt <= gk4[r] && lex_lt(kt, rowi[t], thk[r], ths[r]);
+          const uint64_t m = __ballot(c);
+          if (m) {
+            if (c) {
+              const uint32_t pos = qcount + __popcll(m & ((1ull << lane) - 1ull));
+              cq[wave][pos] = ((uint64_t)rowi[t] << 8) | (uint64_t)(g + 4 * r);
+              cqk[wave][pos] = kt;
+            }
+            qcount += __popcll(m);
+            if (qcount >= 64) {
+              qcount -= 64;
+              run_batch(qcount, 64);
+            }
+          }
         }
-      }
-      if (moved) {
-        if (lane == j) {
-          tk_l = tkj;
-          ts_l = tsj;
-          // a full list publishes its k-th key
-          if (a.gtk && tkj != ~0ull && tkj < gk_l)
-            __hip_atomic_fetch_min(a.gtk + qw + j, tkj, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      uint64_t key[NT][4];
+      bool hit = false;
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const double sc = METRIC == kCos ? -(acc[t][r] / (qn4[r] * xn[t])) : -acc[t][r];
+          key[t][r] = rank_bits(sc);
+          hit |= qv4[r] && rv[t] && key[t][r] <= gk4[r] &&
+                 lex_lt(key[t][r], rowi[t], thk[r], ths[r]);
         }
-        if (mine) {
-          thk[r] = tkj;
-          ths[r] = tsj;
+      if (__builtin_expect(__ballot(hit) == 0, 1)) return;
+#pragma unroll
+      for (int j = 0; j < kScanTQ; j++) {
+        const int r = j >> 2;
+        const bool mine = g == (j & 3);
+        if (qw + j >= a.nq) break;
+        uint64_t tkj = read_lane64(tk_l, j);
+        uint32_t tsj = __builtin_amdgcn_readlane(ts_l, j);
+        const int kqj = __builtin_amdgcn_readlane(kq_l, j);
+        const uint64_t gkj = read_lane64(gk_l, j);
+        bool moved = false;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          uint64_t m = __ballot(mine && rv[t] && key[t][r] <= gkj &&
+                                lex_lt(key[t][r], rowi[t], tkj, tsj));
+          while (m) {
+            const int src = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            L[j].insert(uniform64(__shfl(key[t][r], src)), base + 16 * t + (src & 15), kqj,
+                        lane);
+            L[j].kth(kqj, tkj, tsj);
+            m &= __ballot(mine && lex_lt(key[t][r], rowi[t], tkj, tsj));
+            moved = true;
+          }
+        }
+        if (moved) {
+          if (lane == j) {
+            tk_l = tkj;
+            ts_l = tsj;
+            // a full list publishes its k-th key
+            if (a.gtk && tkj != ~0ull && tkj < gk_l)
+              __hip_atomic_fetch_min(a.gtk + qw + j, tkj, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+          }
+          if (mine) {
+            thk[r] = tkj;
+            ths[r] = tsj;
+          }
         }
       }
     }
@@ -833,9 +872,7 @@ k_scan_f64_mfma(ScanF64Args a) {
     stage(buf0, buf1, st0);
     if (st0 + 1 < nstages) stage(buf1, buf0, st0 + 1);
   }
-  if constexpr (METRIC == kL2) {
-    if (active && qcount) run_batch(0, qcount);
-  }
+  if (active && qcount) run_batch(0, qcount);
 
   if (!active) return;
   const uint32_t list = slice;
