@@ -638,7 +638,7 @@ k_scan_f64_mfma(ScanF64Args a) {
 #pragma unroll
   for (int j = 0; j < kScanTQ; j++) L[j].init();
   // lane j: query qw + j's global bound, read once per row step
-  uint64_t gk_l = ~0ull;
+  uint64_t gk_l = ~0ull, gk_next = ~0ull;
 
   f64x4 acc[NT];
 #pragma unroll
@@ -709,9 +709,7 @@ k_scan_f64_mfma(ScanF64Args a) {
       xn[t] = METRIC != kDot ? a.st.xn[slot[t]] : 0.0;
       xx[t] = METRIC == kL2 ? a.st.nx2[slot[t]] : 0.0;
     }
-    if (a.gtk && lane < kScanTQ)
-      gk_l = __hip_atomic_load(a.gtk + min(qw + lane, qlast), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    gk_l = gk_next;
     uint64_t gk4[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) gk4[r] = __shfl(gk_l, g + 4 * r);
@@ -838,6 +836,11 @@ k_scan_f64_mfma(ScanF64Args a) {
   auto stage = [&](const unsigned char* cur, unsigned char* nxt, uint32_t st)
                    __attribute__((always_inline)) {
     const uint32_t step = st / nchunks, chunk = st % nchunks;
+    // the global bounds for this row step's epilogue, read at its first
+    // stage so the load's latency runs under the step's MFMAs
+    if (chunk == 0 && a.gtk && lane < kScanTQ)
+      gk_next = __hip_atomic_load(a.gtk + min(qw + lane, qlast), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int s = 0; s < KS; s++) qa[s] = qn[s];
     if (st + 1 < nstages) {
