@@ -9,8 +9,7 @@ at what that design could get wrong: non-finite and overflowing values
 (the bound turns non-finite), subnormal products, exact ties and zero
 distances, large offsets where (xx + qq) − 2·dot cancels almost every bit,
 deleted rows, stores shorter than one 64-row stage, partial query groups,
-and k on both sides of 64 (one or two list registers per lane).  Every batch here
-holds at least 64 queries: smaller ones take the VALU scan.
+and k on both sides of 64 (one or two list registers per lane).
 """
 import sys
 
@@ -100,7 +99,7 @@ def test_matrix_core_short_stores_and_deletes(n):
     rng = np.random.default_rng(100 + n)
     D = 32
     X = rng.standard_normal((n, D))
-    Q = rng.standard_normal((70, D))          # (the matrix cores take batches of 64 and up)
+    Q = rng.standard_normal((70, D))
     say(f"matrix-core scan, {n} rows")
     db = svdb.Database(kd_dimension=D)
     db.bulk_insert(X)
