@@ -8,9 +8,10 @@
 // key: the bound of svdb_lowp.h holds with this shadow's ρ (≈ 2^-5 instead
 // of ≈ 2^-9).  Queries are quantized the same way, with their own scale sq.
 //
-// Scan (k_scan_wide): the structure of k_scan_bf16 (8 wavefronts, queries held as B
-// operands, row tiles streamed direct-to-LDS into two static buffers, XCD
-// owning whole slices, per-lane 8-entry sorted lists) with
+// Scan (k_scan_wide): queries held as B operands, row tiles streamed
+// direct-to-LDS into two static buffers, XCD owning whole slices, per-lane
+// 8-entry sorted lists, a branch-free per-fragment test among the next
+// subtile's MFMAs and one rare wave-uniform insertion branch, with
 // v_mfma_scale_f32_16x16x128_f8f6f4 (unit E8M0 scales): lane l supplies
 // row (or query) l%16, bytes k = 32·(l/16) .. +31 of each 128-byte K step;
 // the result has rows 4·(l/16)+r for query l%16.  Ordering keys (2 VALU ops):
@@ -386,21 +387,34 @@ __global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
     }
   }
 
-  auto epilogue = [&](const f32x4 (&acc)[QF], const float4& w0, const float4& w1, uint32_t srow) {
-    const float v0[4] = {w0.x, w0.y, w0.z, w0.w};
-    const float v1[4] = {w1.x, w1.y, w1.z, w1.w};
+  auto key = [&](const f32x4 (&acc)[QF], int f, int r, const float4& w0, const float4& w1) {
+    const float v0 = r == 0 ? w0.x : r == 1 ? w0.y : r == 2 ? w0.z : w0.w;
+    const float v1 = r == 0 ? w1.x : r == 1 ? w1.y : r == 2 ? w1.z : w1.w;
+    return BF ? (COS ? fmaf(acc[f][r], v0, v1) : fmaf(P[f], v0, acc[f][r]))
+              : (COS ? fmaf(acc[f][r] * P[f], v0, v1) : fmaf(P[f], v0, acc[f][r] * v1));
+  };
+  // Fragment f's common-case test: does any of its four keys beat the
+  // list's last entry?  Branch-free VALU work, issued among the next
+  // subtile's MFMAs (the minimum-only pre-pass folds its keys here).
+  auto need_of = [&](const f32x4 (&acc)[QF], int f, const float4& w0, const float4& w1) {
+    const float mn = fminf(fminf(key(acc, f, 0, w0, w1), key(acc, f, 1, w0, w1)),
+                           fminf(key(acc, f, 2, w0, w1), key(acc, f, 3, w0, w1)));
+    if constexpr (MINONLY) {
+      tau[f] = fminf(tau[f], mn);
+      return false;
+    }
+    return mn < tau[f];
+  };
+  // The rare insertion path recomputes the keys.
+  auto insert_all = [&](const f32x4 (&acc)[QF], const float4& w0, const float4& w1,
+                        uint32_t srow) {
 #pragma unroll
     for (int f = 0; f < QF; f++) {
       float k4[4];
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        k4[r] = BF ? (COS ? fmaf(acc[f][r], v0[r], v1[r]) : fmaf(P[f], v0[r], acc[f][r]))
-                   : (COS ? fmaf(acc[f][r] * P[f], v0[r], v1[r])
-                          : fmaf(P[f], v0[r], acc[f][r] * v1[r]));
+      for (int r = 0; r < 4; r++) k4[r] = key(acc, f, r, w0, w1);
       const float mn = fminf(fminf(k4[0], k4[1]), fminf(k4[2], k4[3]));
-      if constexpr (MINONLY) {
-        tau[f] = fminf(tau[f], mn);
-      } else if (mn < tau[f]) {
+      if (mn < tau[f]) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           if (k4[r] < lk[f][LP - 1]) {
@@ -422,14 +436,19 @@ __global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
       }
     }
   };
+  auto epilogue = [&](const f32x4 (&acc)[QF], const float4& w0, const float4& w1, uint32_t srow) {
+    bool need = false;
+#pragma unroll
+    for (int f = 0; f < QF; f++) need |= need_of(acc, f, w0, w1);
+    if (!MINONLY && __ballot(need) != 0) insert_all(acc, w0, w1, srow);
+  };
 
-  // The previous subtile's accumulators and per-row values: its epilogue
-  // runs after the next subtile's first MFMAs have issued, so the key and
-  // list work (VALU) overlaps MFMAs in flight instead of leaving the
-  // matrix core idle — with one wavefront per SIMD nothing else covers it.
-  // The initial values make every key +inf (Euclidean: P > 0 with w0 =
-  // +inf; dot: P < 0 with w0 = -inf; cosine: w1 = +inf), so the first
-  // epilogue inserts nothing.
+  // The previous subtile's accumulators and per-row values: its test runs
+  // among the next subtile's MFMAs, so the key work (VALU) overlaps MFMAs in
+  // flight instead of leaving the matrix core idle — with one wavefront per
+  // SIMD nothing else covers it.  The initial values make every key +inf
+  // (Euclidean: P > 0 with w0 = +inf; dot: P < 0 with w0 = -inf; cosine:
+  // w1 = +inf), so the first test passes nothing.
   f32x4 pacc[QF];
 #pragma unroll
   for (int f = 0; f < QF; f++) pacc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -461,19 +480,29 @@ __global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
         const i32x4 hi = *reinterpret_cast<const i32x4*>(cur + G::chunk(arow, c1) * 16);
         return i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
       };
+      // The A operands stream through two register buffers across the
+      // tile's subtiles: after global step g's MFMAs have issued, its buffer
+      // takes step g+2's operand, so each read has a whole step of MFMAs to
+      // arrive, and a subtile's first steps do not wait on LDS.
+      constexpr int NSUB = G::TR / 16;
+      static_assert(KS % 2 == 0, "the operand buffers alternate by K step");
+      i32x8 Ab[2];
+      if (active) {
+        Ab[0] = loadA(ql, 0);
+        Ab[1] = loadA(ql, 1);
+      }
 #pragma unroll
-      for (int sub = 0; sub < (active ? G::TR / 16 : 0); sub++) {
-        const int arow = sub * 16 + ql;
+      for (int sub = 0; sub < (active ? NSUB : 0); sub++) {
         f32x4 acc[QF];
 #pragma unroll
         for (int f = 0; f < QF; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // Two operand buffers alternate: after step ks's MFMAs have issued,
-        // its buffer takes step ks+2's operand, so each read has a whole
-        // step of MFMAs to arrive (measured 2 % on the 768-wide FP8 scan
-        // against reading step ks+1 there).
-        i32x8 Ab[2];
-        Ab[0] = loadA(arow, 0);
-        if (KS > 1) Ab[1] = loadA(arow, 1);
+        // The previous subtile's test, one fragment per K step (the rest at
+        // the last step): VALU work in the same basic block as this
+        // subtile's MFMAs, issued between them — with one wavefront per
+        // SIMD nothing else would fill the matrix core meanwhile.  One
+        // wave-uniform, unlikely branch after the last step takes the rare
+        // insertions out of line.
+        bool need = false;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
           const i32x8 A = Ab[ks & 1];
@@ -492,16 +521,21 @@ __global__ __launch_bounds__(64 * NW) void k_scan_wide(LowpScanArgs a) {
                                                                         0, 127, 0, 127);
             }
           }
-          if (ks + 2 < KS) {
-            __builtin_amdgcn_sched_barrier(0);
-            Ab[ks & 1] = loadA(arow, ks + 2);
+#pragma unroll
+          for (int f = 0; f < QF; f++)
+            if (f == ks || (ks == KS - 1 && f > ks)) need |= need_of(pacc, f, pw0, pw1);
+          const int nx = sub * KS + ks + 2;
+          if (nx < NSUB * KS) Ab[ks & 1] = loadA((nx / KS) * 16 + ql, nx % KS);
+          // one MFMA, then up to four VALU instructions, and so on
+#pragma unroll
+          for (int i = 0; i < QF * (BF ? 2 : 1); i++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
           }
           __builtin_amdgcn_sched_barrier(0);
-          if (ks == 0) {
-            epilogue(pacc, pw0, pw1, prow);
-            __builtin_amdgcn_sched_barrier(0);
-          }
         }
+        if (!MINONLY && __builtin_expect(__ballot(need) != 0, 0))
+          insert_all(pacc, pw0, pw1, prow);
         const int rl = sub * 16 + rg * 4;
 #pragma unroll
         for (int f = 0; f < QF; f++) pacc[f] = acc[f];
