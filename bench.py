@@ -810,7 +810,9 @@ def main():
         "mfma_sustained": {
             "unit": "TFLOP/s", **sustained,
             "method": "svdb_mfma_sustained: independent MFMAs (BF16 16x16x32, FP8 16x16x128, "
-                      "FP64 16x16x4), no memory traffic, 2 and 4 wavefronts per SIMD, best of 3",
+                      "FP64 16x16x4) with their accumulators in VGPRs as in the scans, operands "
+                      "fixed in registers, no memory traffic, 2 and 4 wavefronts per SIMD, best "
+                      "of 3",
             "headline_kernel_frac": (round(achieved / sustained["bf16"], 4)
                                      if sustained.get("bf16") else None),
         },
