@@ -590,6 +590,12 @@ k_scan_f64_mfma(ScanF64Args a) {
   constexpr int QCAP = METRIC == kL2 ? 64 + 16 * 16 * NT : (KPL == 1 ? 128 : 1);
   __shared__ uint64_t cq[kWavesPerBlock][QCAP];
   __shared__ uint64_t cqk[kWavesPerBlock][METRIC == kL2 ? 1 : QCAP];
+  // Per-row values of a row step (‖x‖, Σx², the alive words), staged
+  // through LDS with the stage loads one row step ahead (two row steps in
+  // flight, by parity), so the epilogue reads them from LDS instead of
+  // waiting on global memory
+  __shared__ __attribute__((aligned(16))) double rowv[2][2][KPL == 1 ? kF64StageRows : 1];
+  __shared__ __attribute__((aligned(16))) uint32_t alivew[2][4];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, jr = lane & 15;
@@ -696,18 +702,52 @@ k_scan_f64_mfma(ScanF64Args a) {
     }
   };
 
+  // The per-row values of the row step at base into rowv / alivew[par]:
+  // wave 0 the norms, wave 1 the squared norms (4-byte direct-to-LDS loads,
+  // lane l: dword l&1 of row 32i + l/2, rows clamped to the store), wave 2
+  // the (at most two) alive words the 64 rows span.
+  // (one list register per lane only: at two the staging's registers cost
+  // spills, and the row step's loads stay direct)
+  constexpr bool ROWV = KPL == 1;
+  auto issue_rowvals = [&](int par, uint32_t base) __attribute__((always_inline)) {
+    if constexpr (!ROWV) return;
+    const uint32_t last = a.st.nrows - 1;
+    if ((wave == 0 && METRIC != kDot) || (wave == 1 && METRIC == kL2)) {
+      const double* src = wave == 0 ? a.st.xn : a.st.nx2;
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        const uint32_t row = min(base + 32 * i + (lane >> 1), last);
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(src + row) + (lane & 1),
+                                         (lds_void_t*)&rowv[par][wave][32 * i], 4, 0, 0);
+      }
+    } else if (wave == 2 && lane < 4) {
+      const uint32_t w = min((base >> 6) + (lane >> 1), last >> 6);
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint32_t*>(a.st.alive + w) + (lane & 1),
+                                       (lds_void_t*)&alivew[par][0], 4, 0, 0);
+    }
+  };
+
   // Per-row-step epilogue: keys of the step's 64 rows, list insertions.
-  auto epilogue = [&](uint32_t base) __attribute__((always_inline)) {
-    uint32_t rowi[NT], slot[NT];
+  auto epilogue = [&](uint32_t base, int par) __attribute__((always_inline)) {
+    uint32_t rowi[NT];
     bool rv[NT];
     double xn[NT], xx[NT];
+    const uint32_t w0 = base >> 6;
 #pragma unroll
     for (int t = 0; t < NT; t++) {
       rowi[t] = base + 16 * t + jr;
-      slot[t] = min(rowi[t], a.st.nrows - 1);
-      rv[t] = rowi[t] < r_end && slot_alive(a.st.alive, slot[t]);
-      xn[t] = METRIC != kDot ? a.st.xn[slot[t]] : 0.0;
-      xx[t] = METRIC == kL2 ? a.st.nx2[slot[t]] : 0.0;
+      if constexpr (ROWV) {
+        const uint32_t wi = (rowi[t] >> 6) - w0;  // 0 or 1
+        const uint64_t word = ((uint64_t)alivew[par][2 * wi + 1] << 32) | alivew[par][2 * wi];
+        rv[t] = rowi[t] < r_end && ((word >> (rowi[t] & 63)) & 1ull);
+        xn[t] = METRIC != kDot ? rowv[par][0][16 * t + jr] : 0.0;
+        xx[t] = METRIC == kL2 ? rowv[par][1][16 * t + jr] : 0.0;
+      } else {
+        const uint32_t slot = min(rowi[t], a.st.nrows - 1);
+        rv[t] = rowi[t] < r_end && slot_alive(a.st.alive, slot);
+        xn[t] = METRIC != kDot ? a.st.xn[slot] : 0.0;
+        xx[t] = METRIC == kL2 ? a.st.nx2[slot] : 0.0;
+      }
     }
     gk_l = gk_next;
     uint64_t gk4[4];
@@ -824,6 +864,7 @@ k_scan_f64_mfma(ScanF64Args a) {
 
   if (nstages > 0) {
     f64_issue_stage(a.st, buf0, r_begin, 0, wave, lane);
+    issue_rowvals(0, r_begin);
     load_q(0);
   }
   __builtin_amdgcn_s_waitcnt(0);
@@ -847,6 +888,7 @@ k_scan_f64_mfma(ScanF64Args a) {
       const uint32_t nstep = (st + 1) / nchunks, nchunk = (st + 1) % nchunks;
       f64_issue_stage(a.st, nxt, r_begin + nstep * kF64StageRows, nchunk * kF64StageCols, wave,
                       lane);
+      if (nchunk == 0) issue_rowvals(nstep & 1, r_begin + nstep * kF64StageRows);
       load_q(nchunk * kF64StageCols);
     }
     if (active) {
@@ -863,7 +905,7 @@ k_scan_f64_mfma(ScanF64Args a) {
           acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[s], b[t], acc[t], 0, 0, 0);
       }
       if (chunk + 1 == nchunks) {
-        epilogue(r_begin + step * kF64StageRows);
+        epilogue(r_begin + step * kF64StageRows, step & 1);
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[t] = (f64x4){0.0, 0.0, 0.0, 0.0};
       }

@@ -117,3 +117,33 @@ def test_matrix_core_short_stores_and_deletes(n):
         for metric in ("l2", "cos", "dot"):
             check(db, Xk, Q, 5, metric, what=f"n={n} after deletes")
     db.close()
+
+
+def test_matrix_core_cosine_underflowing_quotients():
+    # rows whose dot product with the query is subnormal or tiny while
+    # ‖q‖·‖x‖ is ~1e11: cos = dot / (‖q‖‖x‖) underflows to subnormals or
+    # zero, so the k-th best key sits in the subnormal range, where a
+    # pre-test that skips the division must still let every candidate in
+    rng = np.random.default_rng(77)
+    D = 32
+    n = 6000
+    X = rng.standard_normal((n, D)) * 1e10
+    tiny = np.array([1e-300, -1e-300, 1e-305, -1e-305, 1e-310, -1e-310, 5e-324, -5e-324,
+                     0.0, -0.0, 2.5e-320, 1e-200, 1e-10])
+    X[:, 0] = tiny[rng.integers(0, len(tiny), n)]
+    Q = rng.standard_normal((70, D))
+    Q[0] = 0.0
+    Q[0, 0] = 1.0
+    Q[1] = 0.0
+    Q[1, 0] = -1.0
+    Q[2] = 0.0
+    Q[2, 0] = 3.0
+    Q[3] = 0.0
+    Q[3, 0] = 1e-300
+    say("matrix-core scan, cosine with underflowing quotients, 6000x32, 70 queries")
+    db = svdb.Database(kd_dimension=D)
+    db.bulk_insert(X)
+    for k in (1, 10, 40, 64):
+        check(db, X, Q, k, "cos", what="underflow")
+        check(db, X, Q, k, "dot", what="underflow")
+    db.close()
