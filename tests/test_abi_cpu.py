@@ -58,3 +58,12 @@ def test_init_refuses_an_overflowing_capacity():
     # capacity * sizeof(Vector) would wrap: refused before any allocation
     # (vector_database.c:17-54 has the same multiplication unchecked)
     assert not svdb.lib().vector_db_init((1 << 64) // 32, 3)
+
+
+def test_mfma_sustained_refuses_bad_arguments():
+    # the kind and the output are checked before the device is touched
+    v = C.c_double(-1.0)
+    assert svdb.lib().svdb_mfma_sustained(3, C.byref(v)) == -1   # SVDB_EINVAL
+    assert svdb.lib().svdb_mfma_sustained(-1, C.byref(v)) == -1
+    assert svdb.lib().svdb_mfma_sustained(0, None) == -1
+    assert b"svdb_mfma_sustained" in svdb.lib().svdb_last_error()
