@@ -179,8 +179,15 @@ size_t kdtree_nearest(KDTree* tree, const double* point) {
   // as long.  Below that the FP64 scan answers (the same ids either way).
   const bool small = (double)e->phys_rows() * (double)D <= (double)(1u << 26);
   const int prec = D == e->dim() && !small ? SVDB_PREC_AUTO : SVDB_PREC_FP64;
-  if (c_call([&] { return e->search_host(point, 1, D, D, &k, &metric, prec, 1, &id, &score); }) !=
-      SVDB_OK) {
+  // small stores: one launch answers (k_nearest1; the same id as the scan)
+  bool handled = false;
+  if (small && c_call([&] { return e->nearest1(point, D, &id, &handled); }) != SVDB_OK) {
+    fprintf(stderr, "kdtree_nearest: %s\n", svdb::last_error());
+    return (size_t)-1;
+  }
+  if (!handled &&
+      c_call([&] { return e->search_host(point, 1, D, D, &k, &metric, prec, 1, &id, &score); }) !=
+          SVDB_OK) {
     fprintf(stderr, "kdtree_nearest: %s\n", svdb::last_error());
     return (size_t)-1;
   }

@@ -156,6 +156,8 @@ Engine::~Engine() {
   void* bufs[] = {X_, nx2_, xn_, alive_, rank_, sh_.Xb, sh_.nx, sh_.xn, sh_.inv, sh_.pen,
                   sh_.rho_bits /* degen_count and degen live inside this block */, sh8_.X8,
                   sh8_.sx, sh8_.nx, sh8_.xn, sh8_.inv, sh8_.pen, sh8_.rho_bits};
+  if (near_dev_) (void)hipFree(near_dev_);
+  if (near_host_) (void)hipHostFree(near_host_);
   for (auto& c : ctx_) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     void* cb[] = {c.s_q_, c.s_qn_, c.s_k_, c.s_pk_, c.s_ps_, c.s_ids_, c.s_sc_, c.s_pair_,
@@ -1519,6 +1521,37 @@ int Engine::search_device_noalloc(const double* dQ, size_t nq, size_t ldq, size_
   }
   st_.searches++;
   st_.queries += nq;
+  return SVDB_OK;
+}
+
+int Engine::nearest1(const double* q, size_t D, size_t* id, bool* handled) {
+  std::lock_guard<std::mutex> g(mu_);
+  *handled = false;
+  if (graphs_on_ || D == 0 || D > kNearestMaxD || D > d_ || nphys_ == 0) return SVDB_OK;
+  c_ = &ctx_[0];
+  int rc;
+  const size_t part_b = kNearestMaxBlocks * (sizeof(uint64_t) + sizeof(uint32_t));
+  if (!near_dev_) {
+    // the ticket starts at 0 and every launch leaves it at 0
+    HIPCHK(hipMalloc(&near_dev_, part_b + 256), "hipMalloc");
+    HIPCHK(hipMemset(static_cast<char*>(near_dev_) + part_b, 0, 256), "hipMemset");
+  }
+  if (!near_host_) HIPCHK(hipHostMalloc((void**)&near_host_, 64), "hipHostMalloc");
+  if ((rc = sync_alive())) return rc;
+  Nearest1Args a{};
+  a.st = StoreView{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0, nullptr};
+  a.D = (uint32_t)D;
+  a.part_key = static_cast<uint64_t*>(near_dev_);
+  a.part_slot = reinterpret_cast<uint32_t*>(a.part_key + kNearestMaxBlocks);
+  a.ticket = reinterpret_cast<uint32_t*>(static_cast<char*>(near_dev_) + part_b);
+  a.out_id = near_host_;
+  for (size_t i = 0; i < D; i++) a.q[i] = q[i];
+  *near_host_ = ~0ull;
+  HIPCHK(launch_nearest1(a, c_->stream), "k_nearest1");
+  HIPCHK(hipStreamSynchronize(c_->stream), "k_nearest1");
+  const uint64_t v = *reinterpret_cast<volatile uint64_t*>(near_host_);
+  *id = v == ~0ull ? (size_t)-1 : (size_t)v;
+  *handled = true;
   return SVDB_OK;
 }
 

@@ -95,6 +95,12 @@ class Engine {
   // Host queries; per-query k and metric; ids/scores row stride ostride.
   int search_host(const double* Q, size_t nq, size_t ldq, size_t D, const size_t* k,
                   const int* metric, int precision, size_t ostride, size_t* ids, double* scores);
+  // kdtree_nearest's one-query form: the nearest live row over the first D
+  // columns in one launch (k_nearest1), the id read back from host memory.
+  // Sets *handled = false, and does nothing, where it does not apply (D
+  // above kNearestMaxD or the row width, or graph replay switched on); the
+  // caller then takes search_host.
+  int nearest1(const double* q, size_t D, size_t* id, bool* handled);
   // Split form of search_host for the serving loop: begin enqueues the
   // search on context `slot` (1 or 2) and returns; end completes it (waits,
   // runs any retries) and writes ids / scores, which must stay valid from
@@ -293,7 +299,8 @@ class Engine {
   SearchCtx* c_ = &ctx_[0];
   void* h_stage_ = nullptr;  // pinned: row upload staging
   size_t h_stage_n_ = 0;
-  int ctx_stream(SearchCtx& c);
+  void* near_dev_ = nullptr;        // k_nearest1: block results and ticket
+  uint64_t* near_host_ = nullptr;   // pinned: its answer
   int lowp_enqueue(const double* dQ, size_t ldq, LowpRange* R, int nr, int k, uint64_t* dids,
                    double* dscores, uint32_t ostride, bool fp8);
   int lowp_complete();

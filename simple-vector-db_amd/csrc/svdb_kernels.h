@@ -82,6 +82,24 @@ uint32_t scan_f64_lists(int metric, uint32_t D, const StoreView& st, int kmax, u
 hipError_t launch_scan_f64(const ScanF64Args& a, int metric, int kmax, uint32_t nslices,
                            hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, int metric, int kmax, hipStream_t s);
+
+// The nearest live row to one query over the first D columns (canonical
+// Euclidean key, lowest slot on ties) in one launch: each block reduces its
+// rows, the last block to finish reduces the blocks' results, writes the
+// row's logical id ((uint64)-1 when no row is live) to out_id — host
+// memory the caller reads after the launch — and puts the ticket back to 0.
+constexpr uint32_t kNearestMaxD = 256;
+constexpr uint32_t kNearestMaxBlocks = 1024;
+struct Nearest1Args {
+  StoreView st;
+  uint32_t D;
+  uint64_t* part_key;   // [kNearestMaxBlocks]
+  uint32_t* part_slot;  // [kNearestMaxBlocks]
+  uint32_t* ticket;     // 0 on entry and on exit
+  uint64_t* out_id;
+  double q[kNearestMaxD];
+};
+hipError_t launch_nearest1(const Nearest1Args& a, hipStream_t s);
 // A loop of independent MFMAs (kind 0 FP64 16x16x4, 1 BF16 16x16x32, 2 FP8
 // 16x16x128) over `blocks` blocks of 4 wavefronts; out holds blocks · 256
 // floats.  mfma_loop_flop gives the flop of one such launch.

@@ -1002,6 +1002,86 @@ hipError_t launch_scan_f64(const ScanF64Args& a, int metric, int kmax, uint32_t 
   return hipErrorInvalidValue;
 }
 
+// ------------------------------------------------ one query, nearest row
+//
+// kdtree_nearest called once per request (the reference's handlers do) on a
+// store small enough to scan in microseconds: the launch, not the scan, is
+// the cost, so the whole answer is one kernel — no query upload (the query
+// travels in the kernel arguments), no merge launch, no result copy (the
+// last block writes the id to host memory).  The blocks' results meet
+// through a ticket: every block publishes its (key, slot) and takes a
+// ticket (release/acquire at agent scope); the block holding the last
+// ticket reads them all.
+
+__device__ __forceinline__ void min_pair(uint64_t& bk, uint32_t& bs, uint64_t k, uint32_t s) {
+  if (lex_lt(k, s, bk, bs)) {
+    bk = k;
+    bs = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nearest1(const Nearest1Args a) {
+  __shared__ uint64_t wk[4];
+  __shared__ uint32_t ws[4];
+  __shared__ int last;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t bk = ~0ull;
+  uint32_t bs = ~0u;
+  for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < a.st.nrows; r += gridDim.x * 256) {
+    if (!slot_alive(a.st.alive, r)) continue;
+    const double* __restrict__ x = a.st.X + (size_t)r * a.st.ldx;
+    double s = 0.0;
+    for (uint32_t i = 0; i < a.D; i++) {
+      const double t = x[i] - a.q[i];
+      s = fma(t, t, s);
+    }
+    min_pair(bk, bs, rank_bits(s), r);
+  }
+  auto block_min = [&]() {
+    for (int off = 32; off >= 1; off >>= 1) {
+      const uint64_t ok = (uint64_t)__shfl_xor((long long)bk, off);
+      const uint32_t os = (uint32_t)__shfl_xor((int)bs, off);
+      min_pair(bk, bs, ok, os);
+    }
+    if (lane == 0) {
+      wk[wave] = bk;
+      ws[wave] = bs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 1; w < 4; w++) min_pair(bk, bs, wk[w], ws[w]);
+  };
+  block_min();
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(a.part_key + blockIdx.x, bk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a.part_slot + blockIdx.x, bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t =
+        __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  bk = ~0ull;
+  bs = ~0u;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+    min_pair(bk, bs, __hip_atomic_load(a.part_key + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+             __hip_atomic_load(a.part_slot + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  __syncthreads();  // wk / ws are reused
+  block_min();
+  if (threadIdx.x == 0) {
+    *a.out_id = bs == ~0u ? ~0ull : logical_id(a.st, bs);
+    __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+hipError_t launch_nearest1(const Nearest1Args& a, hipStream_t s) {
+  if (a.D == 0 || a.D > kNearestMaxD || a.st.nrows == 0) return hipErrorInvalidValue;
+  const uint32_t blocks =
+      std::min<uint32_t>(kNearestMaxBlocks, (a.st.nrows + 255) / 256);
+  hipLaunchKernelGGL(k_nearest1, dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------- list merge
 //
 // One wavefront per query merges the query's P sorted partial lists into

@@ -52,10 +52,21 @@ def test_search_allocation_failure_then_recovers():
         db.search(Q, 5, "l2", "fp64")
     assert e.value.code == ENOMEM
     assert svdb.fail_host_alloc(-1) == 0  # fired
-    # the reference entry point reports its sentinel
+    # kdtree_nearest's one-launch path (small store, kd dimension <= 256)
+    # allocates no host memory: it answers, and the hook stays armed
     svdb.fail_host_alloc(1)
-    assert db.kd_nearest(Q[0]) == svdb.NOID
+    rid, _ = O.topk(X, Q[:1], 1, "l2", D=64)
+    assert db.kd_nearest(Q[0]) == int(rid[0, 0])
+    assert svdb.fail_host_alloc(-1) == 1  # not fired
     svdb.fail_host_alloc(0)
+    # on the batched scan (kd dimension above that path's 256 columns) the
+    # reference entry point reports its sentinel
+    wide = svdb.Database(kd_dimension=300)
+    wide.bulk_insert(rng.standard_normal((500, 300)))
+    svdb.fail_host_alloc(1)
+    assert wide.kd_nearest(rng.standard_normal(300)) == svdb.NOID
+    svdb.fail_host_alloc(0)
+    wide.close()
     for m in ("l2", "cos", "dot"):
         _exact(db, X, Q, 5, m, "fp64")
         _exact(db, X, Q, 5, m, "bf16")
