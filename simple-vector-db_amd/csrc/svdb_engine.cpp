@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1536,7 +1537,10 @@ int Engine::nearest1(const double* q, size_t D, size_t* id, bool* handled) {
     HIPCHK(hipMalloc(&near_dev_, part_b + 256), "hipMalloc");
     HIPCHK(hipMemset(static_cast<char*>(near_dev_) + part_b, 0, 256), "hipMemset");
   }
-  if (!near_host_) HIPCHK(hipHostMalloc((void**)&near_host_, 64), "hipHostMalloc");
+  if (!near_host_) {
+    HIPCHK(hipHostMalloc((void**)&near_host_, 64), "hipHostMalloc");
+    near_host_[0] = near_host_[1] = 0;  // seq starts at 1
+  }
   if ((rc = sync_alive())) return rc;
   Nearest1Args a{};
   a.st = StoreView{X_, ld_, xn_, alive_, rank_, (uint32_t)nphys_, has_deletes_ ? 1 : 0, nullptr};
@@ -1545,11 +1549,26 @@ int Engine::nearest1(const double* q, size_t D, size_t* id, bool* handled) {
   a.part_slot = reinterpret_cast<uint32_t*>(a.part_key + kNearestMaxBlocks);
   a.ticket = reinterpret_cast<uint32_t*>(static_cast<char*>(near_dev_) + part_b);
   a.out_id = near_host_;
+  a.seq = ++near_seq_;
   for (size_t i = 0; i < D; i++) a.q[i] = q[i];
-  *near_host_ = ~0ull;
   HIPCHK(launch_nearest1(a, c_->stream), "k_nearest1");
-  HIPCHK(hipStreamSynchronize(c_->stream), "k_nearest1");
-  const uint64_t v = *reinterpret_cast<volatile uint64_t*>(near_host_);
+  // The answer is in host memory as soon as seq is: poll it (a stream wait
+  // costs several microseconds more than the kernel); after 50 ms without
+  // it, wait for the stream, which also reports a failed launch.
+  volatile uint64_t* hv = near_host_;
+  const auto t0 = std::chrono::steady_clock::now();
+  while (hv[1] != a.seq) {
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      HIPCHK(hipStreamSynchronize(c_->stream), "k_nearest1");
+      if (hv[1] != a.seq) {
+        set_error(SVDB_EDEVICE, "k_nearest1: no answer");
+        return SVDB_EDEVICE;
+      }
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  const uint64_t v = hv[0];
   *id = v == ~0ull ? (size_t)-1 : (size_t)v;
   *handled = true;
   return SVDB_OK;

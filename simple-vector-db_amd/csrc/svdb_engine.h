@@ -300,7 +300,8 @@ class Engine {
   void* h_stage_ = nullptr;  // pinned: row upload staging
   size_t h_stage_n_ = 0;
   void* near_dev_ = nullptr;        // k_nearest1: block results and ticket
-  uint64_t* near_host_ = nullptr;   // pinned: its answer
+  uint64_t* near_host_ = nullptr;   // pinned: its answer (id, seq)
+  uint64_t near_seq_ = 0;
   int lowp_enqueue(const double* dQ, size_t ldq, LowpRange* R, int nr, int k, uint64_t* dids,
                    double* dscores, uint32_t ostride, bool fp8);
   int lowp_complete();

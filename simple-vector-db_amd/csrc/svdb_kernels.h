@@ -96,7 +96,8 @@ struct Nearest1Args {
   uint64_t* part_key;   // [kNearestMaxBlocks]
   uint32_t* part_slot;  // [kNearestMaxBlocks]
   uint32_t* ticket;     // 0 on entry and on exit
-  uint64_t* out_id;
+  uint64_t* out_id;     // [2]: the id, then seq once the id is visible
+  uint64_t seq;
   double q[kNearestMaxD];
 };
 hipError_t launch_nearest1(const Nearest1Args& a, hipStream_t s);

@@ -1069,8 +1069,10 @@ __global__ __launch_bounds__(256) void k_nearest1(const Nearest1Args a) {
   __syncthreads();  // wk / ws are reused
   block_min();
   if (threadIdx.x == 0) {
-    *a.out_id = bs == ~0u ? ~0ull : logical_id(a.st, bs);
     __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a.out_id[0] = bs == ~0u ? ~0ull : logical_id(a.st, bs);
+    __threadfence_system();
+    __hip_atomic_store(a.out_id + 1, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
