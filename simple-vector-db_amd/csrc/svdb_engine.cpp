@@ -1283,8 +1283,16 @@ int Engine::lowp_complete() {
     // 1024, then 4096, from the same candidate lists (when they hold at
     // least twice that many entries) and reranked; only those still
     // uncertified go on.
-    for (uint32_t Lw : {1024u, (uint32_t)kMaxShort}) {
+    for (int wi = 0; wi < 2; wi++) {
+      const uint32_t Lw = wi == 0 ? 1024u : (uint32_t)kMaxShort;
       if (nf == 0 || Lw <= L || (size_t)NL * kLowpLP < 2 * (size_t)Lw) continue;
+      // A retry that has lately certified almost none of its queries for
+      // this precision and metric is skipped (its queries go on to the next
+      // stage), except every 16th time, which keeps the estimate current.
+      // Which stage certifies a query never changes its ids.
+      float& miss = retry_miss_[fp8 ? 1 : 0][metric][wi];
+      if (miss > 0.95f && (++retry_seen_[fp8 ? 1 : 0][metric][wi] % 16) != 0) continue;
+      const uint32_t nf_in = nf;
       if ((rc = scratch(&c_->s_sk2_, &c_->s_sk2_n_, (size_t)nf * Lw * sizeof(float)))) return rc;
       if ((rc = scratch(&c_->s_ss2_, &c_->s_ss2_n_, (size_t)nf * Lw * sizeof(uint32_t)))) return rc;
       if ((rc = scratch(&c_->s_beta2_, &c_->s_beta2_n_, (size_t)nf * sizeof(float)))) return rc;
@@ -1316,6 +1324,11 @@ int Engine::lowp_complete() {
       HIPCHK(hipStreamSynchronize(c_->stream), "lowp widened");
       nf = *c_->h_fail_;
       fail_list = out_list;
+      // (a batch of 256 or more failing queries is a sample on its own)
+      if (nf_in >= 32) {
+        const float w = nf_in >= 256 ? 1.0f : 0.5f;
+        miss = (1.0f - w) * miss + w * (float)nf / (float)nf_in;
+      }
     }
     // the error report again, now with the widened queries certified
     if (nf != nf1) {

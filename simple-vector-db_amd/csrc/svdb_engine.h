@@ -194,6 +194,10 @@ class Engine {
   std::vector<uint32_t> h_rank_;
   bool has_deletes_ = false;
   bool degen_dirty_ = false;  // degenerate lists may hold deleted / rewritten slots
+  // The widening retries' recent share of queries left uncertified, per
+  // precision (BF16, FP8), metric and width (1024, 4096); see lowp_complete.
+  float retry_miss_[2][3][2] = {};
+  uint32_t retry_seen_[2][3][2] = {};
   size_t dirty_word_ = SIZE_MAX;  // first bitmap word not yet on the device
 
   // low-precision shadow
