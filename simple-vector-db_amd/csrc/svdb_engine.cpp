@@ -95,11 +95,13 @@ struct RefCtx {
   size_t cap = 0;  // doubles
   uint32_t* slots = nullptr;
   float* out = nullptr;
+  uint64_t* host = nullptr;  // pinned: the one-launch form's (result, seq)
+  uint64_t seq = 0;
 };
 RefCtx g_ref;
 }  // namespace
 
-int pair_ref32_host(const double* a, const double* b, size_t n, int metric, float* out) {
+int pair_ref32_host(const double* a_in, const double* b_in, size_t n, int metric, float* out) {
   if (device_init() < 0) return SVDB_EDEVICE;
   std::lock_guard<std::mutex> g(g_ref.mu);
   if (!g_ref.stream) {
@@ -108,6 +110,38 @@ int pair_ref32_host(const double* a, const double* b, size_t n, int metric, floa
     HIPCHK(hipMalloc(&g_ref.out, sizeof(float)), "hipMalloc");
     uint32_t s[2] = {0, 1};
     HIPCHK(hipMemcpy(g_ref.slots, s, sizeof(s), hipMemcpyHostToDevice), "copy");
+    HIPCHK(hipHostMalloc((void**)&g_ref.host, 64), "hipHostMalloc");
+    g_ref.host[0] = g_ref.host[1] = 0;
+  }
+  if (n <= kRef32ArgsMax) {
+    // one launch: the pair in the kernel arguments, the result polled from
+    // pinned host memory (a stream wait after 50 ms reports a failed launch)
+    PairRef32Args a;
+    a.n = (uint32_t)n;
+    a.metric = metric;
+    a.out = g_ref.host;
+    a.seq = ++g_ref.seq;
+    for (size_t i = 0; i < n; i++) {
+      a.ab[i] = a_in[i];
+      a.ab[n + i] = b_in[i];
+    }
+    HIPCHK(launch_pair_ref32_args(a, g_ref.stream), "k_pair_ref32_args");
+    volatile uint64_t* hv = g_ref.host;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (hv[1] != a.seq) {
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+        HIPCHK(hipStreamSynchronize(g_ref.stream), "k_pair_ref32_args");
+        if (hv[1] != a.seq) {
+          set_error(SVDB_EDEVICE, "k_pair_ref32_args: no answer");
+          return SVDB_EDEVICE;
+        }
+        break;
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    const uint32_t bits = (uint32_t)hv[0];
+    std::memcpy(out, &bits, sizeof(float));
+    return SVDB_OK;
   }
   size_t ld = std::max<size_t>(n, 1);
   if (2 * ld > g_ref.cap) {
@@ -117,8 +151,8 @@ int pair_ref32_host(const double* a, const double* b, size_t n, int metric, floa
     g_ref.cap = 2 * ld;
   }
   if (n) {
-    HIPCHK(hipMemcpyAsync(g_ref.buf, a, n * sizeof(double), hipMemcpyHostToDevice, g_ref.stream), "copy");
-    HIPCHK(hipMemcpyAsync(g_ref.buf + ld, b, n * sizeof(double), hipMemcpyHostToDevice, g_ref.stream), "copy");
+    HIPCHK(hipMemcpyAsync(g_ref.buf, a_in, n * sizeof(double), hipMemcpyHostToDevice, g_ref.stream), "copy");
+    HIPCHK(hipMemcpyAsync(g_ref.buf + ld, b_in, n * sizeof(double), hipMemcpyHostToDevice, g_ref.stream), "copy");
   }
   HIPCHK(launch_pair_ref32(g_ref.buf, ld, g_ref.slots, g_ref.slots + 1, 1, (uint32_t)n, metric,
                            g_ref.out, g_ref.stream), "k_pair_ref32");

@@ -110,5 +110,17 @@ hipError_t launch_pair_f64(const double* X, size_t ldx, const uint32_t* sa, cons
                            uint32_t np, uint32_t D, int metric, double* out, hipStream_t s);
 hipError_t launch_pair_ref32(const double* M, size_t ld, const uint32_t* sa, const uint32_t* sb,
                              uint32_t np, uint32_t n, int metric, float* out, hipStream_t s);
+// One pair of at most kRef32ArgsMax elements each, carried in the kernel
+// arguments (a then b in ab), the reference's float metric written to
+// host memory: out[0] the float's bits, then out[1] = seq.
+constexpr uint32_t kRef32ArgsMax = 224;
+struct PairRef32Args {
+  uint32_t n;
+  int metric;
+  uint64_t* out;  // [2], host memory
+  uint64_t seq;
+  double ab[2 * kRef32ArgsMax];
+};
+hipError_t launch_pair_ref32_args(const PairRef32Args& a, hipStream_t s);
 
 }  // namespace svdb

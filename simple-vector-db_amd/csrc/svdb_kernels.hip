@@ -1226,14 +1226,11 @@ hipError_t launch_pair_f64(const double* X, size_t ldx, const uint32_t* sa, cons
 // to the reference: FP64 products added into float accumulators
 // (acc = (float)((double)acc + a*b)); Euclidean squares the float-rounded
 // difference in float; sqrt and the cosine division in double.
-__global__ void k_pair_ref32(const double* __restrict__ M, size_t ld,
-                             const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
-                             uint32_t np, uint32_t n, int metric, float* __restrict__ out) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const double* A = M + (size_t)sa[p] * ld;
-  const double* B = M + (size_t)sb[p] * ld;
-  float r;
+// The reference's float metric of one pair (vector_database.c:301-352):
+// float accumulators, each step widened to double and rounded back, as its
+// C compiles.
+__device__ float ref32_metric(const double* __restrict__ A, const double* __restrict__ B,
+                              uint32_t n, int metric) {
   if (metric == kCos) {
     float dp = 0.0f, na = 0.0f, nb = 0.0f;
     for (uint32_t i = 0; i < n; i++) {
@@ -1241,21 +1238,44 @@ __global__ void k_pair_ref32(const double* __restrict__ M, size_t ld,
       na = (float)((double)na + A[i] * A[i]);
       nb = (float)((double)nb + B[i] * B[i]);
     }
-    r = (float)((double)dp / (sqrt((double)na) * sqrt((double)nb)));
-  } else if (metric == kL2) {
+    return (float)((double)dp / (sqrt((double)na) * sqrt((double)nb)));
+  }
+  if (metric == kL2) {
     float sum = 0.0f;
     for (uint32_t i = 0; i < n; i++) {
       float diff = (float)(A[i] - B[i]);
       float sq = diff * diff;
       sum = sum + sq;
     }
-    r = (float)sqrt((double)sum);
-  } else {
-    float acc = 0.0f;
-    for (uint32_t i = 0; i < n; i++) acc = (float)((double)acc + A[i] * B[i]);
-    r = acc;
+    return (float)sqrt((double)sum);
   }
-  out[p] = r;
+  float acc = 0.0f;
+  for (uint32_t i = 0; i < n; i++) acc = (float)((double)acc + A[i] * B[i]);
+  return acc;
+}
+
+__global__ void k_pair_ref32(const double* __restrict__ M, size_t ld,
+                             const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
+                             uint32_t np, uint32_t n, int metric, float* __restrict__ out) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  out[p] = ref32_metric(M + (size_t)sa[p] * ld, M + (size_t)sb[p] * ld, n, metric);
+}
+
+// One pair from the kernel arguments (the drop-in per-call functions): the
+// result, then seq once it is visible, into host memory the caller polls.
+__global__ void k_pair_ref32_args(const PairRef32Args a) {
+  if (threadIdx.x != 0) return;
+  const float r = ref32_metric(a.ab, a.ab + a.n, a.n, a.metric);
+  a.out[0] = (uint64_t)__float_as_uint(r);
+  __threadfence_system();
+  __hip_atomic_store(a.out + 1, a.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_pair_ref32_args(const PairRef32Args& a, hipStream_t s) {
+  if (a.n > kRef32ArgsMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pair_ref32_args, dim3(1), dim3(64), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_pair_ref32(const double* M, size_t ld, const uint32_t* sa, const uint32_t* sb,
