@@ -245,6 +245,7 @@ class Engine {
     void* s_beta_ = nullptr;   size_t s_beta_n_ = 0;
     void* s_fail_ = nullptr;   size_t s_fail_n_ = 0;
     void* s_err_ = nullptr;    size_t s_err_n_ = 0;
+    void* s_tau_ = nullptr;    size_t s_tau_n_ = 0;   // uncertified queries' k-th values
     void* s_fq_ = nullptr;     size_t s_fq_n_ = 0;
     void* s_fid_ = nullptr;    size_t s_fid_n_ = 0;
     void* s_fsc_ = nullptr;    size_t s_fsc_n_ = 0;

@@ -139,6 +139,14 @@ struct RerankArgs {
   uint32_t* fail_list;      // their indices
   double* max_rel_err;      // [nq] first-pass score error of the returned scores
   const uint32_t* qidx;     // optional: shortlist row i is query qidx[i]
+  // Optional, indexed by query q.  tau_in[q] is the FP64 ordering value of
+  // the k-th entry of a list of k actual rows (an earlier stage's), so the
+  // true k-th lies at or below it: a shortlisted row whose first-pass lower
+  // bound exceeds it cannot be in the top k and is not fetched.  A query
+  // left uncertified writes its own list's k-th value to tau_out[q] (which
+  // may be tau_in: each query's entry is read and written by one wavefront).
+  const double* tau_in;
+  double* tau_out;
 };
 
 // Queries per block of the scan for a padded row length dp.

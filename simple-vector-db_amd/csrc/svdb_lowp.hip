@@ -647,6 +647,20 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
   const uint32_t ndeg = min(*a.degen_count, (uint32_t)kDegenCap);
   const int kk = (int)a.k;
 
+  // FP64 lower bound on the ordering value of a row whose first-pass key is
+  // at least `bo` (the rounding slack of the key's conversion included)
+  auto lower = [&](float bo) __attribute__((always_inline)) -> double {
+    if (bo == INFINITY) return INFINITY;
+    const double bs = (double)bo * qp.scale;
+    double bR = bs + qp.off;
+    bR -= 0x1p-40 * (fabs(bs) + fabs(qp.off));
+    if (METRIC == kL2 && bR > 0.0) bR *= 1.0 - (double)(a.d + 4) * 0x1p-52;
+    return bR;
+  };
+  // a flagged query's keys carry no bound: no pruning
+  const bool prune = !qp.flags;
+  const double tin = (a.tau_in && prune) ? a.tau_in[q] : INFINITY;
+
   WL<2> L;
   L.init();
   uint64_t tk = ~0ull;
@@ -659,6 +673,14 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
     else if (j < ncand) slot = a.degen[j - a.L];
     bool valid = slot != ~0u && slot < a.st.nrows &&
                  ((a.st.alive[slot >> 6] >> (slot & 63)) & 1ull);
+    // Prune threshold: the earlier stage's k-th value or this list's, the
+    // smaller (+inf: fetch every candidate).  A row whose lower bound exceeds
+    // it cannot be in the top k.  (A NaN bound compares false and keeps its
+    // row.)
+    const double thr = prune ? fmin(tin, ts == ~0u ? INFINITY : rank_val64(tk)) : INFINITY;
+    if (valid && j < a.L && thr < INFINITY && lower(a.short_key[(size_t)i * a.L + j]) > thr)
+      valid = false;
+    if (__ballot(valid) == 0) continue;
     // rows this lane fetches: candidates 4u + lane/16
     const double* src[16];
 #pragma unroll
@@ -727,24 +749,13 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
   // certification
   // more degenerate rows than the list holds: nothing can be certified
   bool cert = false;
-  if (!qp.flags && *a.degen_count <= (uint32_t)kDegenCap) {
-    const float bo = a.beta[i];
-    double bR;
-    if (bo == INFINITY) {
-      bR = INFINITY;
-    } else {
-      const double bs = (double)bo * qp.scale;
-      bR = bs + qp.off;
-      bR -= 0x1p-40 * (fabs(bs) + fabs(qp.off));
-      if (METRIC == kL2 && bR > 0.0) bR *= 1.0 - (double)(a.d + 4) * 0x1p-52;
-    }
-    const double tau = ts == ~0u ? INFINITY : rank_val64(tk);
-    cert = tau < bR;
-  }
+  const double tau = ts == ~0u ? INFINITY : rank_val64(tk);
+  if (!qp.flags && *a.degen_count <= (uint32_t)kDegenCap) cert = tau < lower(a.beta[i]);
   if (!cert) {
     if (lane == 0) {
       uint32_t idx = atomicAdd(a.fail_count, 1u);
       a.fail_list[idx] = q;
+      if (a.tau_out) a.tau_out[q] = fmin(tau, tin);
     }
     return;
   }

@@ -231,3 +231,43 @@ def test_fp8_wide_workload_shape(n, d):
         say(f"  certified {cert} (stages counted as they certify: {wid} only after a wider "
             f"retry; {sec} handed to the BF16 pass, counted there again), FP64 rescans {fb}")
     db.close()
+
+
+def test_fp8_retries_with_exact_ties():
+    """Widening retries prune shortlisted rows by lower bounds (the FP8 key's,
+    then the BF16 row's) against an earlier list's k-th value.  Rows here are
+    small integers, exact in e4m3 (so the keys' rounding slack is all that
+    separates a bound from the value), copied ~100 times each: the top 100
+    of a query near a base row is its copies, whose shared value is the
+    k-th, ties broken by id, and a query whose copies outnumber the
+    128-wide shortlist goes on to the wider retries.  Searched before and
+    after deletes and updates; every answer equals the oracle's."""
+    rng = np.random.default_rng(71)
+    d = 768
+    nb = 2000
+    base = rng.integers(-8, 9, size=(nb, d)).astype(np.float64)
+    X = base[rng.integers(0, nb, size=200_000)]
+    db = store(X)
+    db.set_shortlist(128)
+    rows = [r for r in X]
+    Q = np.vstack([base[rng.integers(0, nb, size=48)] + 0.25 * rng.standard_normal((48, d)),
+                   base[rng.integers(0, nb, size=16)]])
+    for phase in ("initial", "mutated"):
+        if phase == "mutated":
+            for i in (11, 500, 120_000, 199_999):
+                db.update(i, base[(i * 7) % nb])
+                rows[i] = base[(i * 7) % nb].copy()
+            for i in sorted(rng.choice(len(rows), 40, replace=False).tolist(), reverse=True):
+                db.delete(i)
+                del rows[i]
+        Xm = np.stack(rows)
+        for metric in ("l2", "cos", "dot"):
+            say(f"fp8 ties {phase} {metric} k=100 L=128")
+            s0 = db.stats()
+            ids, sc = db.search(Q, 100, metric, precision="fp8")
+            s1 = db.stats()
+            rid, rsc = O.topk(Xm, Q, 100, metric)
+            assert_same(ids, sc, rid, rsc, f"fp8 ties {phase} {metric}")
+            say(f"  widened {s1['widened'] - s0['widened']}, second pass "
+                f"{s1['second_pass'] - s0['second_pass']}, fp64 {s1['fallbacks'] - s0['fallbacks']}")
+    db.close()
