@@ -197,7 +197,7 @@ Engine::~Engine() {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     void* cb[] = {c.s_q_, c.s_qn_, c.s_k_, c.s_pk_, c.s_ps_, c.s_ids_, c.s_sc_, c.s_pair_,
                   c.s_pout_, c.s_qb_, c.s_qp_, c.s_ck_, c.s_cs_, c.s_sk_, c.s_ss_, c.s_beta_,
-                  c.s_fail_, c.s_err_, c.s_tau_, c.s_fq_, c.s_fid_, c.s_fsc_, c.s_fk_, c.s_sk2_, c.s_ss2_,
+                  c.s_fail_, c.s_err_, c.s_lk_, c.s_ls_, c.s_km_, c.s_fq_, c.s_fid_, c.s_fsc_, c.s_fk_, c.s_sk2_, c.s_ss2_,
                   c.s_beta2_, c.s_fail2_, c.s_cidx_, c.s_cq_, c.s_cid_, c.s_csc_};
     for (void* p : cb)
       if (p) (void)hipFree(p);
@@ -1093,7 +1093,10 @@ int Engine::lowp_enqueue(const double* dQ, size_t ldq, LowpRange* R, int nr, int
   // fail counts (one per range) then each range's fail list at nr + soff
   if ((rc = scratch(&c_->s_fail_, &c_->s_fail_n_, ((size_t)nq + nr) * sizeof(uint32_t)))) return rc;
   if ((rc = scratch(&c_->s_err_, &c_->s_err_n_, (size_t)nq * sizeof(double)))) return rc;
-  if ((rc = scratch(&c_->s_tau_, &c_->s_tau_n_, (size_t)nq * sizeof(double)))) return rc;
+  // (a range's lists sit at soff * k, k the largest over ranges)
+  if ((rc = scratch(&c_->s_lk_, &c_->s_lk_n_, (size_t)nq * k * sizeof(uint64_t)))) return rc;
+  if ((rc = scratch(&c_->s_ls_, &c_->s_ls_n_, (size_t)nq * k * sizeof(uint32_t)))) return rc;
+  if ((rc = scratch(&c_->s_km_, &c_->s_km_n_, (size_t)nq * sizeof(float)))) return rc;
   if (!c_->h_fail_) HIPCHK(hipHostMalloc((void**)&c_->h_fail_, sizeof(uint32_t)), "hipHostMalloc");
   const size_t io_b = round_up(nr * sizeof(uint32_t), 64) + (size_t)nq * sizeof(double);
   if (c_->h_io_n_ < io_b) {
@@ -1213,8 +1216,9 @@ int Engine::lowp_enqueue(const double* dQ, size_t ldq, LowpRange* R, int nr, int
                       w1_of(rg.metric), rg.n, width(rg.k), (uint32_t)rg.k, ostride,
                       dids + (size_t)rg.qoff * ostride, dscores + (size_t)rg.qoff * ostride,
                       (uint32_t*)c_->s_fail_ + ri, const_cast<uint32_t*>(fl),
-                      (double*)c_->s_err_ + rg.soff, nullptr, nullptr,
-                      (double*)c_->s_tau_ + rg.soff};
+                      (double*)c_->s_err_ + rg.soff, nullptr,
+                      (uint64_t*)c_->s_lk_ + (size_t)rg.soff * k,
+                      (uint32_t*)c_->s_ls_ + (size_t)rg.soff * k, (float*)c_->s_km_ + rg.soff, 0};
   };
   std::vector<RerankArgs> ras(nr);
   std::vector<int> rms(nr);
@@ -1285,8 +1289,10 @@ int Engine::lowp_complete() {
                       w1_of(rg.metric), rg.n, width(rg.k), (uint32_t)rg.k, ostride,
                       dids + (size_t)rg.qoff * ostride, dscores + (size_t)rg.qoff * ostride,
                       (uint32_t*)c_->s_fail_ + ri, const_cast<uint32_t*>(fl),
-                      (double*)c_->s_err_ + rg.soff, nullptr, nullptr,
-                      (double*)c_->s_tau_ + rg.soff};
+                      (double*)c_->s_err_ + rg.soff, nullptr,
+                      (uint64_t*)c_->s_lk_ + (size_t)rg.soff * lp.k,
+                      (uint32_t*)c_->s_ls_ + (size_t)rg.soff * lp.k, (float*)c_->s_km_ + rg.soff,
+                      0};
   };
   uint32_t* h_counts = static_cast<uint32_t*>(c_->h_io_);
   double* h_err = reinterpret_cast<double*>(static_cast<char*>(c_->h_io_) +
@@ -1353,8 +1359,9 @@ int Engine::lowp_complete() {
       rw.fail_count = out_count;
       rw.fail_list = out_list;
       rw.qidx = fail_list;
-      // the earlier stage's k-th values prune this shortlist's fetches
-      rw.tau_in = rw.tau_out;
+      // the list starts from the earlier stage's; its candidates are not
+      // read again
+      rw.carry = 1;
       prof_begin(SVDB_K_RERANK);
       HIPCHK(launch_rerank(rw, metric, c_->stream), "k_rerank (widened)");
       prof_end(SVDB_K_RERANK);

@@ -245,7 +245,9 @@ class Engine {
     void* s_beta_ = nullptr;   size_t s_beta_n_ = 0;
     void* s_fail_ = nullptr;   size_t s_fail_n_ = 0;
     void* s_err_ = nullptr;    size_t s_err_n_ = 0;
-    void* s_tau_ = nullptr;    size_t s_tau_n_ = 0;   // uncertified queries' k-th values
+    void* s_lk_ = nullptr;     size_t s_lk_n_ = 0;    // uncertified queries' lists: keys
+    void* s_ls_ = nullptr;     size_t s_ls_n_ = 0;    //   slots
+    void* s_km_ = nullptr;     size_t s_km_n_ = 0;    //   shortlists' largest key
     void* s_fq_ = nullptr;     size_t s_fq_n_ = 0;
     void* s_fid_ = nullptr;    size_t s_fid_n_ = 0;
     void* s_fsc_ = nullptr;    size_t s_fsc_n_ = 0;

@@ -139,14 +139,17 @@ struct RerankArgs {
   uint32_t* fail_list;      // their indices
   double* max_rel_err;      // [nq] first-pass score error of the returned scores
   const uint32_t* qidx;     // optional: shortlist row i is query qidx[i]
-  // Optional, indexed by query q.  tau_in[q] is the FP64 ordering value of
-  // the k-th entry of a list of k actual rows (an earlier stage's), so the
-  // true k-th lies at or below it: a shortlisted row whose first-pass lower
-  // bound exceeds it cannot be in the top k and is not fetched.  A query
-  // left uncertified writes its own list's k-th value to tau_out[q] (which
-  // may be tau_in: each query's entry is read and written by one wavefront).
-  const double* tau_in;
-  double* tau_out;
+  // Optional carried state, indexed by query q: a stage's top k (FP64
+  // ordering keys lk[q*k ..], slots ls[q*k ..]) and kmax[q], the largest
+  // first-pass key of its shortlist and the earlier ones (every candidate
+  // keyed below it was scored there, or pruned as unable to enter the top
+  // k).  A query left uncertified writes them; with carry != 0 a stage
+  // starts its list from them and skips candidates keyed below kmax.  Each
+  // query's entries are read and written by one wavefront.
+  uint64_t* lk;
+  uint32_t* ls;
+  float* kmax;
+  int carry;
 };
 
 // Queries per block of the scan for a padded row length dp.

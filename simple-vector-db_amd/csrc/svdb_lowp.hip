@@ -659,12 +659,28 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
   };
   // a flagged query's keys carry no bound: no pruning
   const bool prune = !qp.flags;
-  const double tin = (a.tau_in && prune) ? a.tau_in[q] : INFINITY;
 
   WL<2> L;
   L.init();
   uint64_t tk = ~0ull;
   uint32_t ts = ~0u;
+  // the earlier stage's list, and the key below which its shortlists
+  // already covered every candidate
+  const bool carry = a.carry && a.lk;
+  float kprev = -INFINITY;
+  if (carry) {
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      const int e = p * 64 + lane;
+      if (e < kk) {
+        L.k[p] = a.lk[(size_t)q * kk + e];
+        L.s[p] = a.ls[(size_t)q * kk + e];
+      }
+    }
+    L.at(kk - 1, tk, ts);
+    kprev = a.kmax[q];
+  }
+  float kmax_lane = -INFINITY;
   const uint32_t ncand = a.L + ndeg;
   for (uint32_t j0 = 0; j0 < ncand; j0 += 64) {
     const uint32_t j = j0 + lane;
@@ -673,13 +689,15 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
     else if (j < ncand) slot = a.degen[j - a.L];
     bool valid = slot != ~0u && slot < a.st.nrows &&
                  ((a.st.alive[slot >> 6] >> (slot & 63)) & 1ull);
-    // Prune threshold: the earlier stage's k-th value or this list's, the
-    // smaller (+inf: fetch every candidate).  A row whose lower bound exceeds
-    // it cannot be in the top k.  (A NaN bound compares false and keeps its
-    // row.)
-    const double thr = prune ? fmin(tin, ts == ~0u ? INFINITY : rank_val64(tk)) : INFINITY;
-    if (valid && j < a.L && thr < INFINITY && lower(a.short_key[(size_t)i * a.L + j]) > thr)
-      valid = false;
+    const float key = j < a.L ? a.short_key[(size_t)i * a.L + j] : -INFINITY;
+    kmax_lane = fmaxf(kmax_lane, key);
+    // scored in an earlier stage (its best k are in the carried list)
+    if (carry && j < a.L && key < kprev) valid = false;
+    // Prune threshold: the list's k-th value once full (+inf: fetch every
+    // candidate).  A row whose lower bound exceeds it cannot be in the top
+    // k.  (A NaN bound compares false and keeps its row.)
+    const double thr = (prune && ts != ~0u) ? rank_val64(tk) : INFINITY;
+    if (valid && j < a.L && thr < INFINITY && lower(key) > thr) valid = false;
     if (__ballot(valid) == 0) continue;
     // rows this lane fetches: candidates 4u + lane/16
     const double* src[16];
@@ -738,7 +756,11 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
     while (m) {
       int src = __ffsll((long long)m) - 1;
       m &= m - 1;
-      L.insert(uni64(__shfl(K, src)), uni32(__shfl(slot, src)), kk, lane);
+      const uint32_t S = uni32(__shfl(slot, src));
+      // a row the carried list already holds (keyed at kprev, or a
+      // degenerate row) must not enter twice
+      if (carry && __ballot(L.s[0] == S || L.s[1] == S) != 0) continue;
+      L.insert(uni64(__shfl(K, src)), S, kk, lane);
       // lanes that no longer beat the (moved) k-th entry drop out
       L.at(kk - 1, tk, ts);
       m &= __ballot(lt2(K, slot, tk, ts));
@@ -752,10 +774,21 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
   const double tau = ts == ~0u ? INFINITY : rank_val64(tk);
   if (!qp.flags && *a.degen_count <= (uint32_t)kDegenCap) cert = tau < lower(a.beta[i]);
   if (!cert) {
+    if (a.lk) {
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        const int e = p * 64 + lane;
+        if (e < kk) {
+          a.lk[(size_t)q * kk + e] = L.k[p];
+          a.ls[(size_t)q * kk + e] = L.s[p];
+        }
+      }
+      for (int off = 32; off >= 1; off >>= 1) kmax_lane = fmaxf(kmax_lane, __shfl_xor(kmax_lane, off));
+      if (lane == 0) a.kmax[q] = fmaxf(kmax_lane, kprev);
+    }
     if (lane == 0) {
       uint32_t idx = atomicAdd(a.fail_count, 1u);
       a.fail_list[idx] = q;
-      if (a.tau_out) a.tau_out[q] = fmin(tau, tin);
     }
     return;
   }
