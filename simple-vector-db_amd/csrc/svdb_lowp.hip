@@ -636,7 +636,8 @@ hipError_t launch_kth_key(const float* key, uint32_t nq, uint32_t n, uint32_t L,
 constexpr int kRC = 16;
 
 template <int METRIC>
-__device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, double* stg) {
+__device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, double* stg,
+                                            uint32_t* queue) {
   const int lane = threadIdx.x & 63;
   const uint32_t i = blk * 4 + (threadIdx.x >> 6);
   if (i >= a.nq) return;
@@ -682,23 +683,10 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
   }
   float kmax_lane = -INFINITY;
   const uint32_t ncand = a.L + ndeg;
-  for (uint32_t j0 = 0; j0 < ncand; j0 += 64) {
-    const uint32_t j = j0 + lane;
-    uint32_t slot = ~0u;
-    if (j < a.L) slot = a.short_slot[(size_t)i * a.L + j];
-    else if (j < ncand) slot = a.degen[j - a.L];
-    bool valid = slot != ~0u && slot < a.st.nrows &&
-                 ((a.st.alive[slot >> 6] >> (slot & 63)) & 1ull);
-    const float key = j < a.L ? a.short_key[(size_t)i * a.L + j] : -INFINITY;
-    kmax_lane = fmaxf(kmax_lane, key);
-    // scored in an earlier stage (its best k are in the carried list)
-    if (carry && j < a.L && key < kprev) valid = false;
-    // Prune threshold: the list's k-th value once full (+inf: fetch every
-    // candidate).  A row whose lower bound exceeds it cannot be in the top
-    // k.  (A NaN bound compares false and keeps its row.)
-    const double thr = (prune && ts != ~0u) ? rank_val64(tk) : INFINITY;
-    if (valid && j < a.L && thr < INFINITY && lower(key) > thr) valid = false;
-    if (__ballot(valid) == 0) continue;
+
+  // Scores up to 64 candidates, lane l's `slot` when `valid`, and enters
+  // them into the list.
+  auto score = [&](uint32_t slot, bool valid) __attribute__((always_inline)) {
     // rows this lane fetches: candidates 4u + lane/16
     const double* src[16];
 #pragma unroll
@@ -713,7 +701,7 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
     // The next chunk's loads go out once this chunk's values sit in LDS
     // (into the same registers), so they are in flight during its FMAs.
     double vals[16];
-    auto fetch = [&](uint32_t c0) {
+    auto fetch = [&](uint32_t c0) __attribute__((always_inline)) {
       const uint32_t e = c0 + (lane & 15);
 #pragma unroll
       for (int u = 0; u < 16; u++) vals[u] = (src[u] && e < a.d) ? src[u][e] : 0.0;
@@ -747,11 +735,6 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
       else if (METRIC == kCos) v = -(acc / (qp.qn * a.st.xn[slot]));
       K = rank_bits64(v);
     }
-    // a degenerate row that is also shortlisted must not enter twice
-    if (valid && j >= a.L) {
-      for (uint32_t t = 0; t < a.L; t++)
-        if (a.short_slot[(size_t)i * a.L + t] == slot) valid = false;
-    }
     uint64_t m = __ballot(valid && lt2(K, slot, tk, ts));
     while (m) {
       int src = __ffsll((long long)m) - 1;
@@ -766,6 +749,50 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
       m &= __ballot(lt2(K, slot, tk, ts));
     }
     L.at(kk - 1, tk, ts);
+  };
+
+  // Candidates that still need their FP64 row are packed, in order, into
+  // this wavefront's queue and scored 64 at a time, so a group fetches 64
+  // rows however many of the shortlist's candidates were skipped.
+  uint32_t nqd = 0;  // queued (wave-uniform; at most 127)
+  uint32_t j0 = 0;
+  for (;;) {
+    while (nqd < 64 && j0 < ncand) {
+      const uint32_t j = j0 + lane;
+      uint32_t slot = ~0u;
+      if (j < a.L) slot = a.short_slot[(size_t)i * a.L + j];
+      else if (j < ncand) slot = a.degen[j - a.L];
+      bool valid = slot != ~0u && slot < a.st.nrows &&
+                   ((a.st.alive[slot >> 6] >> (slot & 63)) & 1ull);
+      const float key = j < a.L ? a.short_key[(size_t)i * a.L + j] : -INFINITY;
+      kmax_lane = fmaxf(kmax_lane, key);
+      // scored in an earlier stage (its best k are in the carried list)
+      if (carry && j < a.L && key < kprev) valid = false;
+      // Prune threshold: the list's k-th value once full (+inf: fetch every
+      // candidate).  A row whose lower bound exceeds it cannot be in the
+      // top k.  (A NaN bound compares false and keeps its row.)
+      const double thr = (prune && ts != ~0u) ? rank_val64(tk) : INFINITY;
+      if (valid && j < a.L && thr < INFINITY && lower(key) > thr) valid = false;
+      // a degenerate row that is also shortlisted must not enter twice
+      if (valid && j >= a.L) {
+        for (uint32_t t = 0; t < a.L; t++)
+          if (a.short_slot[(size_t)i * a.L + t] == slot) valid = false;
+      }
+      const uint64_t vm = __ballot(valid);
+      if (valid) queue[nqd + __popcll(vm & ((1ull << lane) - 1ull))] = slot;
+      nqd += __popcll(vm);
+      j0 += 64;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (nqd == 0) break;
+    const uint32_t n = min(nqd, 64u);
+    const uint32_t slot = (uint32_t)lane < n ? queue[lane] : ~0u;
+    const uint32_t rest = (uint32_t)lane + 64 < nqd ? queue[lane + 64] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    if ((uint32_t)lane + 64 < nqd) queue[lane] = rest;
+    nqd -= n;
+    __builtin_amdgcn_wave_barrier();
+    score(slot, (uint32_t)lane < n);
   }
 
   // certification
@@ -859,13 +886,15 @@ __device__ __forceinline__ void rerank_body(const RerankArgs& a, uint32_t blk, d
 // one wavefront per shortlist row; each range its own metric
 __global__ __launch_bounds__(256) void k_rerank(RangeBatch<RerankArgs> b) {
   __shared__ double stage[4][64 * (kRC + 1)];
+  __shared__ uint32_t queue[4][128];
   const int i = range_of(b, blockIdx.x);
   const uint32_t blk = blockIdx.x - b.first[i];
   double* stg = stage[threadIdx.x >> 6];
+  uint32_t* qu = queue[threadIdx.x >> 6];
   switch (b.metric[i]) {
-    case kL2: rerank_body<kL2>(b.r[i], blk, stg); break;
-    case kCos: rerank_body<kCos>(b.r[i], blk, stg); break;
-    default: rerank_body<kDot>(b.r[i], blk, stg); break;
+    case kL2: rerank_body<kL2>(b.r[i], blk, stg, qu); break;
+    case kCos: rerank_body<kCos>(b.r[i], blk, stg, qu); break;
+    default: rerank_body<kDot>(b.r[i], blk, stg, qu); break;
   }
 }
 
@@ -877,8 +906,10 @@ template <int METRIC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rerank1(
     RangeBatch<RerankArgs> b) {
   __shared__ double stage[4][64 * (kRC + 1)];
+  __shared__ uint32_t queue[4][128];
   const int i = range_of(b, blockIdx.x);
-  rerank_body<METRIC>(b.r[i], blockIdx.x - b.first[i], stage[threadIdx.x >> 6]);
+  rerank_body<METRIC>(b.r[i], blockIdx.x - b.first[i], stage[threadIdx.x >> 6],
+                      queue[threadIdx.x >> 6]);
 }
 
 hipError_t launch_rerank_ranges(const RerankArgs* a, const int* metric, int n, hipStream_t s) {
