@@ -204,3 +204,34 @@ def test_device_entry_point(mix):
     svdb.device_sync()
     rid, rsc = O.topk(X, Q, 10, "dot")
     assert_same(dids.to_host(), dsc.to_host(), rid, rsc, "device")
+
+
+def test_mixed_ranges_retries_with_exact_ties():
+    """Widening retries inside a mixed-metric call.  Rows are small integers
+    (exact in BF16), each copied ~500 times, more than a 256-wide shortlist
+    holds, so a query near a base row cannot be certified before a retry
+    covers all its copies.  Each retry starts from the earlier stage's list
+    and re-scores only the copies keyed exactly at that stage's largest key
+    (all of them here), which must not enter the list twice; each metric
+    range keeps its own lists and k.  Every answer equals the oracle's."""
+    rng = np.random.default_rng(81)
+    d, nb = 128, 400
+    base = rng.integers(-8, 9, size=(nb, d)).astype(np.float64)
+    X = base[rng.integers(0, nb, size=200_000)]
+    db = svdb.Database(kd_dimension=3)
+    db.bulk_insert(X)
+    nq = 192
+    Q = np.vstack([base[rng.integers(0, nb, size=nq - 32)] +
+                   0.25 * rng.standard_normal((nq - 32, d)),
+                   base[rng.integers(0, nb, size=32)]])
+    ks = rng.choice([10, 100], size=nq)
+    ms = rng.choice(["l2", "cos", "dot"], size=nq)
+    say("bf16 mixed ranges over ~500 copies per row, k in {10, 100}")
+    s0 = db.stats()
+    ids, sc = db.search(Q, ks, list(ms), "bf16")
+    s1 = db.stats()
+    _check_mixed(X, Q, ks, ms, ids, sc)
+    wid = s1["widened"] - s0["widened"]
+    say(f"  widened {wid}, fp64 {s1['fallbacks'] - s0['fallbacks']}")
+    assert wid > 0
+    db.close()
